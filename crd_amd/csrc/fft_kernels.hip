@@ -1,0 +1,600 @@
+// HIP kernels of the plain FP32 batched FFT path for gfx950 (CDNA4, wave64) and their launchers.
+//
+//   k_rows     one or more contiguous N-point transforms per workgroup, N <= 16384: coalesced loads straight into
+//              the Stockham core's slots, radix-E register stages with LDS exchanges, coalesced stores. One HBM
+//              read and one write per point. (The reference's single-transform engines: Stockham, in-place-3,
+//              standalone-hier, deep-split; engine/numerics/hesap-fft/kernels/fft_kernels.cpp:187-585, 1603-1670.)
+//   k_cols     column transforms of a row-major n1 x ncols matrix, C adjacent columns per workgroup (C*8 bytes of
+//              every row per load), optionally times the four-step twiddle W_N^{n2 k1} from two FP64-built tables
+//              (the reference's four-step phase 1 and its factored twiddle, fft_kernels.cpp:2363-, fft.hpp:859-891),
+//              and the element-major batch (fft.hpp:462-470, fft_kernels.cpp:593-784).
+//   k_rowtr    row transforms with the four-step's transposed store out[k1 + n1 k2], R rows per workgroup so every
+//              store is a run of R*8 bytes (the reference's phase 2 + scatter).
+//   k_roots*   FP64 table builders (the reference builds W_n in f64 and narrows, fft.hpp:251-260).
+//   k_ref_*    the radix-2 reference transform (fft.hpp:417-460), a differential check only.
+//
+// Every kernel guards its own range (t < count, column < ncols) and runs its barriers unconditionally, so a
+// partial last workgroup is safe.
+#include "fft_core.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace crdfft
+{
+using namespace dev;
+
+namespace
+{
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD b % 8). Renumber them so that each XCD
+// owns one contiguous range of logical blocks, in dispatch order: neighbouring tiles (which share 128-byte lines
+// when a tile row is 64 bytes) then meet in the same L2.
+__device__ __forceinline__ uint64_t xcd_block()
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t nb = gridDim.x;
+    const uint32_t full = nb & ~7u;
+    if (b < full)
+    {
+        return static_cast<uint64_t>(b & 7u) * (full >> 3) + (b >> 3);
+    }
+    return b;
+}
+
+constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_rows
+// ---------------------------------------------------------------------------------------------------------------
+template <int N, int E> struct RowCfg
+{
+    static constexpr int T = N / E;
+    static constexpr int M = T >= 256 ? 1 : 256 / T;
+    static constexpr int threads = T * M;
+    static constexpr int LF = lds_floats(N);
+    // per-transform region: re then im, rounded to a 64-bank row and offset so that the transforms sharing a
+    // wave land on different banks
+    static constexpr int RS = (N == E) ? 0 : round_up(2 * LF, 64) + (T < 64 ? T : 0);
+    static constexpr int lds_bytes = RS * M * 4;
+};
+
+template <int N, int E, bool INV>
+__global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
+{
+    using C = RowCfg<N, E>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int local = tid / C::T;
+    const int i = tid - local * C::T;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * C::M + local;
+    const bool active = t < a.count;
+    cf v[E];
+    if (active)
+    {
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = src[i + C::T * m];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = cf{0.0f, 0.0f};
+        }
+    }
+    const LdsView<1> view{lds + local * C::RS, C::LF};
+    fft_core<N, E, INV, 1>(v, i, view, reinterpret_cast<const cf*>(a.tw));
+    if (active)
+    {
+        cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            dst[i + C::T * m] = v[m];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_cols
+// ---------------------------------------------------------------------------------------------------------------
+template <int N1, int E, int CC> struct ColCfg
+{
+    static constexpr int T = N1 / E;
+    static constexpr int threads = T * CC;
+    static constexpr int LF = lds_floats(N1);
+    static constexpr int IM = LF * CC + 32; // im array after the re array
+    static constexpr int lds_bytes = (N1 == E) ? 0 : (IM + LF * CC) * 4;
+};
+
+template <int N1, int E, int CC, bool INV>
+__global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a)
+{
+    using C = ColCfg<N1, E, CC>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int c = tid % CC;
+    const int i = tid / CC;
+    const uint64_t groups = (a.ncols + CC - 1) / CC;
+    const uint64_t lb = xcd_block();
+    const uint64_t t = lb / groups;
+    const uint64_t col = (lb - t * groups) * CC + c;
+    const bool active = t < a.count && col < a.ncols;
+    cf v[E];
+    if (active)
+    {
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist + col;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = src[static_cast<uint64_t>(i + C::T * m) * a.ncols];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = cf{0.0f, 0.0f};
+        }
+    }
+    const LdsView<CC> view{lds + c, C::IM};
+    fft_core<N1, E, INV, CC>(v, i, view, reinterpret_cast<const cf*>(a.tw));
+    if (!active)
+    {
+        return;
+    }
+    if (a.thi != nullptr)
+    {
+        const cf* thi = reinterpret_cast<const cf*>(a.thi);
+        const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+        const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            const uint64_t e = col * static_cast<uint64_t>(i + C::T * m);
+            const cf w = cmul(thi[e >> a.hshift], tlo[e & lmask]);
+            v[m] = ctw<INV>(v[m], w);
+        }
+    }
+    cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        dst[static_cast<uint64_t>(i + C::T * m) * a.ncols] = v[m];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_rowtr
+// ---------------------------------------------------------------------------------------------------------------
+template <int N2, int E, int RR> struct RowTrCfg
+{
+    static constexpr int T = N2 / E;
+    static constexpr int threads = T * RR;
+    static constexpr int LF = lds_floats(N2);
+    static constexpr int RS = (N2 == E) ? 0 : round_up(2 * LF, 64) + (T < 64 ? T : 0);
+    static constexpr int fft_floats = RS * RR;
+    static constexpr int TP = RR + 1;              // transpose row pitch (odd: conflict-free column writes)
+    static constexpr int tr_floats = 2 * N2 * TP;  // re then im
+    static constexpr int lds_bytes = (fft_floats > tr_floats ? fft_floats : tr_floats) * 4;
+};
+
+template <int N2, int E, int RR, bool INV>
+__global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrArgs a)
+{
+    using C = RowTrCfg<N2, E, RR>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int r = tid / C::T;
+    const int i = tid - r * C::T;
+    const uint64_t groups = a.n1 / RR;
+    const uint64_t lb = xcd_block();
+    const uint64_t t = lb / groups;
+    const uint64_t g = lb - t * groups;
+    const bool active = t < a.count;
+    cf v[E];
+    if (active)
+    {
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = src[i + C::T * m];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = cf{0.0f, 0.0f};
+        }
+    }
+    const LdsView<1> view{lds + r * C::RS, C::LF};
+    fft_core<N2, E, INV, 1>(v, i, view, reinterpret_cast<const cf*>(a.tw));
+    // fft_core's last LDS read is behind a barrier (or it used no LDS): the buffer is free for the transpose
+    if constexpr (N2 == E)
+    {
+        __syncthreads();
+    }
+    float* tre = lds;
+    float* tim = lds + N2 * C::TP;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        const int k2 = i + C::T * m;
+        tre[k2 * C::TP + r] = v[m].x;
+        tim[k2 * C::TP + r] = v[m].y;
+    }
+    __syncthreads();
+    if (!active)
+    {
+        return;
+    }
+    const int r2 = tid % RR;
+    const int j2 = tid / RR;
+    cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        const int k2 = j2 + C::T * m;
+        dst[static_cast<uint64_t>(k2) * a.n1] = cf{tre[k2 * C::TP + r2], tim[k2 * C::TP + r2]};
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// FP64 table builders
+// ---------------------------------------------------------------------------------------------------------------
+// W_d^e as floats from FP64: the angle is reduced to (-d/2, d/2] before sincospi, so it is exact in units of pi.
+__device__ __forceinline__ cf root_fp64(uint64_t e, uint64_t d)
+{
+    e %= d;
+    int64_t s = static_cast<int64_t>(e);
+    if (2 * e > d)
+    {
+        s -= static_cast<int64_t>(d);
+    }
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(s) / static_cast<double>(d), &sn, &cs);
+    return cf{static_cast<float>(cs), static_cast<float>(-sn)};
+}
+
+__global__ void k_roots_lin(cf* out, uint64_t count, uint64_t mult, uint64_t d)
+{
+    const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < count)
+    {
+        out[j] = root_fp64((j * mult) % d, d);
+    }
+}
+
+// Stage table of stage P (entries k*R + j, k < P, j < R): W_{PR}^{j k}.
+__global__ void k_roots_stage(cf* out, uint64_t p, uint64_t r)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx < p * r)
+    {
+        const uint64_t k = idx / r;
+        const uint64_t j = idx - k * r;
+        out[idx] = root_fp64(j * k, p * r);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Radix-2 reference and copies
+// ---------------------------------------------------------------------------------------------------------------
+__global__ void k_ref_bitrev(const cf* in, cf* out, uint64_t n, uint32_t lg, uint64_t count, uint64_t idist,
+                             uint64_t odist)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n * count)
+    {
+        return;
+    }
+    const uint64_t t = idx / n;
+    const uint64_t i = idx - t * n;
+    const uint64_t r = lg == 0 ? 0 : (__brevll(i) >> (64 - lg));
+    out[t * odist + r] = in[t * idist + i];
+}
+
+template <bool INV>
+__global__ void k_ref_stage(cf* data, const cf* w, uint64_t n, uint64_t len, uint64_t count, uint64_t dist)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t half_n = n / 2;
+    if (idx >= half_n * count)
+    {
+        return;
+    }
+    const uint64_t t = idx / half_n;
+    const uint64_t bfly = idx - t * half_n;
+    const uint64_t half = len / 2;
+    const uint64_t grp = bfly / half;
+    const uint64_t k = bfly - grp * half;
+    cf* base = data + t * dist + grp * len;
+    const cf tw = w[k * (n / len)];
+    const cf b = ctw<INV>(base[k + half], tw);
+    const cf av = base[k];
+    base[k] = cadd(av, b);
+    base[k + half] = csub(av, b);
+}
+
+__global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64_t idist, uint64_t odist)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n * count)
+    {
+        return;
+    }
+    const uint64_t t = idx / n;
+    const uint64_t i = idx - t * n;
+    out[t * odist + i] = in[t * idist + i];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// instantiation tables
+// ---------------------------------------------------------------------------------------------------------------
+#define CRDFFT_ROW_LIST(X)                                                                                             \
+    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(32, 32) X(64, 16) X(64, 64) X(128, 16) X(256, 16) X(512, 16)           \
+        X(512, 32) X(1024, 16) X(1024, 32) X(2048, 16) X(2048, 32) X(4096, 16) X(4096, 32) X(4096, 64) X(8192, 16)     \
+            X(8192, 32) X(16384, 16) X(16384, 32)
+
+#define CRDFFT_COL_LIST(X)                                                                                             \
+    X(2, 2, 64) X(4, 4, 64) X(8, 8, 64) X(16, 16, 64) X(16, 16, 16) X(32, 32, 64) X(32, 16, 16) X(64, 64, 64)          \
+        X(64, 16, 16) X(128, 16, 16) X(128, 16, 8) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16) X(512, 16, 8)           \
+            X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+
+#define CRDFFT_ROWTR_LIST(X)                                                                                           \
+    X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16)              \
+        X(512, 16, 8) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+
+constexpr uint64_t kMaxItems = 0xFFFFFFFFull; // work-items per launch
+
+template <typename K> hipError_t set_lds(K kernel, unsigned bytes)
+{
+    if (bytes > 64 * 1024)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(bytes));
+    }
+    return hipSuccess;
+}
+
+} // namespace
+
+bool row_kernel_shape(uint64_t n, int e, LaunchShape* out)
+{
+#define X(N, E)                                                                                                        \
+    if (n == N && e == E)                                                                                              \
+    {                                                                                                                  \
+        out->threads = RowCfg<N, E>::threads;                                                                          \
+        out->per_group = RowCfg<N, E>::M;                                                                              \
+        out->lds_bytes = RowCfg<N, E>::lds_bytes;                                                                      \
+        return true;                                                                                                   \
+    }
+    CRDFFT_ROW_LIST(X)
+#undef X
+    return false;
+}
+
+bool col_kernel_shape(uint64_t n1, int e, int c, LaunchShape* out)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+    {                                                                                                                  \
+        out->threads = ColCfg<N, E, CC>::threads;                                                                      \
+        out->per_group = CC;                                                                                           \
+        out->lds_bytes = ColCfg<N, E, CC>::lds_bytes;                                                                  \
+        return true;                                                                                                   \
+    }
+    CRDFFT_COL_LIST(X)
+#undef X
+    return false;
+}
+
+bool rowtr_kernel_shape(uint64_t n2, int e, int r, LaunchShape* out)
+{
+#define X(N, E, RR)                                                                                                    \
+    if (n2 == N && e == E && r == RR)                                                                                  \
+    {                                                                                                                  \
+        out->threads = RowTrCfg<N, E, RR>::threads;                                                                    \
+        out->per_group = RR;                                                                                           \
+        out->lds_bytes = RowTrCfg<N, E, RR>::lds_bytes;                                                                \
+        return true;                                                                                                   \
+    }
+    CRDFFT_ROWTR_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStream_t s)
+{
+#define X(N, E)                                                                                                        \
+    if (n == N && e == E)                                                                                              \
+    {                                                                                                                  \
+        using C = RowCfg<N, E>;                                                                                        \
+        auto kern = inverse ? k_rows<N, E, true> : k_rows<N, E, false>;                                                \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t max_groups = kMaxItems / C::threads;                                                            \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            RowArgs part = a;                                                                                          \
+            const uint64_t left = a.count - done;                                                                      \
+            const uint64_t groups = std::min<uint64_t>((left + C::M - 1) / C::M, max_groups);                         \
+            part.count = std::min<uint64_t>(left, groups * C::M);                                                      \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, part);   \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_ROW_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+    {                                                                                                                  \
+        using C = ColCfg<N, E, CC>;                                                                                    \
+        auto kern = inverse ? k_cols<N, E, CC, true> : k_cols<N, E, CC, false>;                                        \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t groups_per = (a.ncols + CC - 1) / CC;                                                           \
+        const uint64_t max_t = std::max<uint64_t>(1, (kMaxItems / C::threads) / groups_per);                          \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            ColArgs part = a;                                                                                          \
+            part.count = std::min<uint64_t>(a.count - done, max_t);                                                    \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            const uint64_t groups = groups_per * part.count;                                                           \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, part);   \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_COL_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs& a, hipStream_t s)
+{
+#define X(N, E, RR)                                                                                                    \
+    if (n2 == N && e == E && r == RR)                                                                                  \
+    {                                                                                                                  \
+        using C = RowTrCfg<N, E, RR>;                                                                                  \
+        if (a.n1 % RR != 0)                                                                                            \
+            return hipErrorInvalidValue;                                                                               \
+        auto kern = inverse ? k_rowtr<N, E, RR, true> : k_rowtr<N, E, RR, false>;                                      \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t groups_per = a.n1 / RR;                                                                         \
+        const uint64_t max_t = std::max<uint64_t>(1, (kMaxItems / C::threads) / groups_per);                          \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            RowTrArgs part = a;                                                                                        \
+            part.count = std::min<uint64_t>(a.count - done, max_t);                                                    \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            const uint64_t groups = groups_per * part.count;                                                           \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, part);   \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_ROWTR_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+uint64_t stage_twiddle_entries(uint64_t n, int e)
+{
+    if (n <= 1 || e <= 1 || static_cast<uint64_t>(e) >= n)
+    {
+        return 0;
+    }
+    const int lg = ilog2c(static_cast<long>(n));
+    const int le = ilog2c(e);
+    const uint64_t r0 = (lg % le) != 0 ? (uint64_t{1} << (lg % le)) : static_cast<uint64_t>(e);
+    uint64_t total = 0;
+    for (uint64_t p = r0; p < n; p *= static_cast<uint64_t>(e))
+    {
+        total += p * static_cast<uint64_t>(e);
+    }
+    return total;
+}
+
+hipError_t build_stage_twiddles(cfloat* table, uint64_t n, int e, hipStream_t s)
+{
+    if (stage_twiddle_entries(n, e) == 0)
+    {
+        return hipSuccess;
+    }
+    const int lg = ilog2c(static_cast<long>(n));
+    const int le = ilog2c(e);
+    const uint64_t r0 = (lg % le) != 0 ? (uint64_t{1} << (lg % le)) : static_cast<uint64_t>(e);
+    uint64_t off = 0;
+    for (uint64_t p = r0; p < n; p *= static_cast<uint64_t>(e))
+    {
+        const uint64_t cnt = p * static_cast<uint64_t>(e);
+        const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
+        hipLaunchKernelGGL(k_roots_stage, dim3(blocks), dim3(256), 0, s, reinterpret_cast<cf*>(table + off), p,
+                           static_cast<uint64_t>(e));
+        off += cnt;
+    }
+    return hipGetLastError();
+}
+
+hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, hipStream_t s)
+{
+    if (count == 0)
+    {
+        return hipSuccess;
+    }
+    const unsigned blocks = static_cast<unsigned>((count + 255) / 256);
+    hipLaunchKernelGGL(k_roots_lin, dim3(blocks), dim3(256), 0, s, reinterpret_cast<cf*>(out), count, mult, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint64_t n, uint64_t count,
+                            uint64_t idist, uint64_t odist, bool inverse, hipStream_t s)
+{
+    if (n == 0 || count == 0)
+    {
+        return hipSuccess;
+    }
+    if (in == out)
+    {
+        return hipErrorInvalidValue; // the bit reversal is out of place
+    }
+    const uint32_t lg = static_cast<uint32_t>(ilog2c(static_cast<long>(n)));
+    const uint64_t items = n * count;
+    if (items > kMaxItems)
+    {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_ref_bitrev, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const cf*>(in), reinterpret_cast<cf*>(out), n, lg, count, idist, odist);
+    const uint64_t bfly = (n / 2) * count;
+    for (uint64_t len = 2; len <= n; len <<= 1)
+    {
+        auto kern = inverse ? k_ref_stage<true> : k_ref_stage<false>;
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((bfly + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<cf*>(out), reinterpret_cast<const cf*>(w), n, len, count, odist);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const cfloat* in, cfloat* out, uint64_t n, uint64_t count, uint64_t idist, uint64_t odist,
+                       hipStream_t s)
+{
+    const uint64_t items = n * count;
+    if (items == 0)
+    {
+        return hipSuccess;
+    }
+    if (items > kMaxItems)
+    {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_copy, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const cf*>(in), reinterpret_cast<cf*>(out), n, count, idist, odist);
+    return hipGetLastError();
+}
+
+} // namespace crdfft

@@ -1,0 +1,90 @@
+// Host-side view of the HIP kernels of the batched FFT path (fft_kernels.hip): launch descriptors and launchers.
+// Every launcher checks its shape against what the kernel assumes and returns false for a shape it has no
+// instantiation for; it never launches a grid the kernel's indexing does not cover.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crdfft
+{
+
+struct cfloat
+{
+    float x, y;
+};
+
+// Contiguous transforms of one size in one workgroup each (or several per workgroup below 256 threads):
+// in/out at t*dist + i, `count` transforms.
+struct RowArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    const cfloat* tw; // stage twiddles of n (stage_twiddle_entries(n, e) entries)
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+};
+
+// Column transforms of length n1 over `ncols` adjacent columns of a row-major n1 x ncols matrix, for `count`
+// matrices at t*dist: four-step pass 1 (with the factored twiddle W_N^{n2 k1} when thi != nullptr) and the
+// element-major batch (ncols = batch, no twiddle).
+struct ColArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    const cfloat* tw;  // stage twiddles of n1
+    const cfloat* thi; // W_N^{j << hshift}
+    const cfloat* tlo; // W_N^{j}, j < 2^hshift
+    uint32_t hshift;
+    uint64_t ncols;
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+};
+
+// Row transforms of length n2 over the n1 rows of a row-major n1 x n2 matrix, stored transposed:
+// element k2 of row k1 goes to out[k1 + n1*k2] (four-step pass 2).
+struct RowTrArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    const cfloat* tw; // stage twiddles of n2
+    uint64_t n1;
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+};
+
+struct LaunchShape
+{
+    unsigned threads = 0;     // workgroup size
+    unsigned per_group = 0;   // rows (transforms, columns) one workgroup covers
+    unsigned lds_bytes = 0;   // dynamic LDS per workgroup
+};
+
+// The values per thread a kernel of size n runs with (E), chosen by the planner among the instantiated ones.
+bool row_kernel_shape(uint64_t n, int e, LaunchShape* out);
+bool col_kernel_shape(uint64_t n1, int e, int c, LaunchShape* out);
+bool rowtr_kernel_shape(uint64_t n2, int e, int r, LaunchShape* out);
+
+hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStream_t s);
+hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
+hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs& a, hipStream_t s);
+
+// Entries of the stage twiddle table of an n-point kernel at e values per thread, and the FP64 device build of it.
+uint64_t stage_twiddle_entries(uint64_t n, int e);
+hipError_t build_stage_twiddles(cfloat* table, uint64_t n, int e, hipStream_t s);
+// out[j] = W_d^{j * mult} for j < count, computed in FP64 on the device and narrowed.
+hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, hipStream_t s);
+
+// The radix-2 reference transform (bit reversal + log2 n butterfly stages over global memory) of `count`
+// contiguous transforms; `w` holds W_n^k for k < n/2.
+hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint64_t n, uint64_t count,
+                            uint64_t idist, uint64_t odist, bool inverse, hipStream_t s);
+
+// out[t*odist + i] = in[t*idist + i] for i < n (n = 1 transforms, and host-side copies of a batch).
+hipError_t launch_copy(const cfloat* in, cfloat* out, uint64_t n, uint64_t count, uint64_t idist, uint64_t odist,
+                       hipStream_t s);
+
+} // namespace crdfft

@@ -1,0 +1,621 @@
+// The planner, the plan object and the C-ABI (include/crdfft.h, include/crdfft_runtime.h) of the batched FFT path.
+//
+// Host code only (C++20, compiled by hipcc for its HIP runtime calls); every transform runs in the HIP kernels of
+// fft_kernels.hip and real_kernels.hip. There is no CPU fallback: a plan that cannot be built on the device is an
+// error returned to the caller.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/crdfft.h"
+#include "../../include/crdfft_runtime.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+#include "real.hpp"
+
+namespace
+{
+
+thread_local int32_t g_last_hip_error = 0;
+
+crdfft_status hip_status(hipError_t e)
+{
+    if (e == hipSuccess)
+    {
+        return CRDFFT_OK;
+    }
+    g_last_hip_error = static_cast<int32_t>(e);
+    if (e == hipErrorOutOfMemory)
+    {
+        return CRDFFT_OUT_OF_MEMORY;
+    }
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    {
+        return CRDFFT_NO_DEVICE;
+    }
+    return CRDFFT_HIP_ERROR;
+}
+
+#define CRDFFT_TRY(expr)                                                                                               \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        const hipError_t crdfft_e_ = (expr);                                                                           \
+        if (crdfft_e_ != hipSuccess)                                                                                   \
+            return hip_status(crdfft_e_);                                                                              \
+    } while (0)
+
+using crdfft::cfloat;
+
+// A device buffer the plan owns.
+struct DeviceBuffer
+{
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+    hipError_t alloc(uint64_t b)
+    {
+        bytes = b;
+        if (b == 0)
+        {
+            return hipSuccess;
+        }
+        return hipMalloc(&ptr, b);
+    }
+    ~DeviceBuffer()
+    {
+        if (ptr != nullptr)
+        {
+            (void)hipFree(ptr);
+        }
+    }
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+} // namespace
+
+struct crdfft_plan_s
+{
+    crdfft::PlanShape shape;
+    uint64_t idist = 0;
+    uint64_t odist = 0;
+    int device = 0;
+    // twiddle tables
+    DeviceBuffer tw_single; // stage table of the single kernel (complex_n)
+    DeviceBuffer tw1, tw2;  // stage tables of the column (n1) and row (n2) kernels
+    DeviceBuffer thi, tlo;  // factored four-step twiddle
+    uint32_t hshift = 0;
+    DeviceBuffer wreal;     // W_n^k, k <= n/2, for R2C/C2R
+    DeviceBuffer wref;      // W_cn^k, k < cn/2, for the radix-2 reference (built on first use)
+    DeviceBuffer workspace; // four-step intermediate, chunk * complex_n
+    DeviceBuffer tw_em;     // stage table of the element-major kernel (built on first use)
+    std::mutex ref_mutex;
+};
+
+namespace
+{
+
+crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
+{
+    const crdfft::PlanShape& sh = p->shape;
+    const uint64_t cn = sh.complex_n;
+    if (sh.algorithm == CRDFFT_ALGO_SINGLE)
+    {
+        const uint64_t entries = crdfft::stage_twiddle_entries(cn, sh.e);
+        CRDFFT_TRY(p->tw_single.alloc(entries * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw_single.as<cfloat>(), cn, sh.e, s));
+    }
+    else if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP)
+    {
+        const uint64_t e1n = crdfft::stage_twiddle_entries(sh.n1, sh.e1);
+        const uint64_t e2n = crdfft::stage_twiddle_entries(sh.n2, sh.e2);
+        CRDFFT_TRY(p->tw1.alloc(e1n * sizeof(cfloat)));
+        CRDFFT_TRY(p->tw2.alloc(e2n * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw1.as<cfloat>(), sh.n1, sh.e1, s));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw2.as<cfloat>(), sh.n2, sh.e2, s));
+        const uint32_t lg = crdfft::log2_exact(cn);
+        p->hshift = lg / 2;
+        const uint64_t lo = uint64_t{1} << p->hshift;
+        const uint64_t hi = cn >> p->hshift;
+        CRDFFT_TRY(p->thi.alloc(hi * sizeof(cfloat)));
+        CRDFFT_TRY(p->tlo.alloc(lo * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), hi, lo, cn, s));
+        CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
+        CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
+    }
+    if (sh.type != CRDFFT_C2C)
+    {
+        const uint64_t h = sh.n / 2;
+        CRDFFT_TRY(p->wreal.alloc((h + 1) * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_roots(p->wreal.as<cfloat>(), h + 1, 1, sh.n, s));
+    }
+    return CRDFFT_OK;
+}
+
+// The complex transform of the plan over `count` transforms (complex distances), on `s`.
+crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint64_t count, uint64_t idist,
+                          uint64_t odist, bool inverse, hipStream_t s)
+{
+    const crdfft::PlanShape& sh = p->shape;
+    const uint64_t cn = sh.complex_n;
+    switch (sh.algorithm)
+    {
+        case CRDFFT_ALGO_COPY:
+            if (in != out)
+            {
+                CRDFFT_TRY(crdfft::launch_copy(in, out, cn, count, idist, odist, s));
+            }
+            return CRDFFT_OK;
+        case CRDFFT_ALGO_SINGLE:
+        {
+            crdfft::RowArgs a{in, out, p->tw_single.as<cfloat>(), count, idist, odist};
+            CRDFFT_TRY(crdfft::launch_rows(cn, sh.e, inverse, a, s));
+            return CRDFFT_OK;
+        }
+        case CRDFFT_ALGO_FOUR_STEP:
+        {
+            for (uint64_t c0 = 0; c0 < count; c0 += sh.chunk)
+            {
+                const uint64_t cnt = std::min(sh.chunk, count - c0);
+                crdfft::ColArgs ca{in + c0 * idist,
+                                   p->workspace.as<cfloat>(),
+                                   p->tw1.as<cfloat>(),
+                                   p->thi.as<cfloat>(),
+                                   p->tlo.as<cfloat>(),
+                                   p->hshift,
+                                   sh.n2,
+                                   cnt,
+                                   idist,
+                                   cn};
+                CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
+                crdfft::RowTrArgs ra{p->workspace.as<cfloat>(), out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
+                                     cn, odist};
+                CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
+            }
+            return CRDFFT_OK;
+        }
+        default:
+            return CRDFFT_UNSUPPORTED;
+    }
+}
+
+crdfft_status create_plan(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist, uint64_t odist,
+                          int32_t type, int32_t precision, uint32_t flags, const crdfft_tuning* tuning)
+{
+    if (plan == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *plan = nullptr;
+    crdfft::PlanShape shape;
+    const crdfft_status st = crdfft::choose_shape(n, batch, type, precision, tuning, &shape);
+    if (st != CRDFFT_OK)
+    {
+        return st;
+    }
+    const uint64_t in_min = type == CRDFFT_C2R ? n / 2 + 1 : n;
+    const uint64_t out_min = type == CRDFFT_R2C ? n / 2 + 1 : n;
+    if (idist == 0)
+    {
+        idist = in_min;
+    }
+    if (odist == 0)
+    {
+        odist = out_min;
+    }
+    if (batch > 1 && (idist < in_min || odist < out_min))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    // the real side is read/written as complex pairs: its distance must be even
+    if ((type == CRDFFT_R2C && (idist % 2) != 0) || (type == CRDFFT_C2R && (odist % 2) != 0))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    {
+        return CRDFFT_NO_DEVICE;
+    }
+    auto* p = new (std::nothrow) crdfft_plan_s;
+    if (p == nullptr)
+    {
+        return CRDFFT_OUT_OF_MEMORY;
+    }
+    p->shape = shape;
+    p->idist = idist;
+    p->odist = odist;
+    (void)hipGetDevice(&p->device);
+    crdfft_status rs = build_tables(p, nullptr);
+    if (rs == CRDFFT_OK)
+    {
+        rs = hip_status(hipStreamSynchronize(nullptr));
+    }
+    if (rs == CRDFFT_OK && (flags & CRDFFT_FLAG_NO_SELF_CHECK) == 0)
+    {
+        rs = crdfft_self_check(p);
+    }
+    if (rs != CRDFFT_OK)
+    {
+        delete p;
+        return rs;
+    }
+    *plan = p;
+    return CRDFFT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+crdfft_status crdfft_plan_create(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist, uint64_t odist,
+                                 int32_t type, int32_t precision, uint32_t flags)
+{
+    return create_plan(plan, n, batch, idist, odist, type, precision, flags, nullptr);
+}
+
+crdfft_status crdfft_plan_create_tuned(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
+                                       uint64_t odist, int32_t type, int32_t precision, uint32_t flags,
+                                       const crdfft_tuning* tuning)
+{
+    return create_plan(plan, n, batch, idist, odist, type, precision, flags, tuning);
+}
+
+crdfft_status crdfft_plan_destroy(crdfft_plan plan)
+{
+    if (plan == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    delete plan;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_execute(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream)
+{
+    if (p == nullptr || in == nullptr || out == nullptr ||
+        (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    const crdfft::PlanShape& sh = p->shape;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool inverse = direction == CRDFFT_INVERSE;
+    if (sh.batch == 0)
+    {
+        return CRDFFT_OK;
+    }
+    if (sh.precision != CRDFFT_FP32)
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    switch (sh.type)
+    {
+        case CRDFFT_C2C:
+            return run_complex(p, static_cast<const cfloat*>(in), static_cast<cfloat*>(out), sh.batch, p->idist,
+                               p->odist, inverse, s);
+        case CRDFFT_R2C:
+        {
+            // rfft is a forward transform: the direction argument must say so
+            if (inverse || in == out)
+            {
+                return CRDFFT_INVALID_ARGUMENT;
+            }
+            const uint64_t h = sh.n / 2;
+            cfloat* o = static_cast<cfloat*>(out);
+            const crdfft_status st =
+                run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist / 2, p->odist, false, s);
+            if (st != CRDFFT_OK)
+            {
+                return st;
+            }
+            CRDFFT_TRY(crdfft::launch_r2c_post(o, p->wreal.as<cfloat>(), h, sh.batch, p->odist, s));
+            return CRDFFT_OK;
+        }
+        case CRDFFT_C2R:
+        {
+            if (!inverse || in == out)
+            {
+                return CRDFFT_INVALID_ARGUMENT;
+            }
+            const uint64_t h = sh.n / 2;
+            cfloat* o = static_cast<cfloat*>(out);
+            CRDFFT_TRY(crdfft::launch_c2r_pre(static_cast<const cfloat*>(in), o, p->wreal.as<cfloat>(), h, sh.batch,
+                                              p->idist, p->odist / 2, 1.0f / static_cast<float>(h), s));
+            return run_complex(p, o, o, sh.batch, p->odist / 2, p->odist / 2, true, s);
+        }
+        default:
+            return CRDFFT_INVALID_ARGUMENT;
+    }
+}
+
+crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, int32_t direction, void* stream)
+{
+    if (p == nullptr || data == nullptr || (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    const crdfft::PlanShape& sh = p->shape;
+    if (sh.type != CRDFFT_C2C || sh.precision != CRDFFT_FP32)
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    if (batch == 0)
+    {
+        return CRDFFT_OK;
+    }
+    if (sh.complex_n == 1)
+    {
+        return CRDFFT_OK;
+    }
+    int e = 0, c = 0;
+    if (!crdfft::element_major_shape(sh.complex_n, batch, &e, &c))
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    // the element-major kernel's stage table: the plan's own when the shapes agree, else built once and kept
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool inverse = direction == CRDFFT_INVERSE;
+    const cfloat* tw = nullptr;
+    if (sh.algorithm == CRDFFT_ALGO_SINGLE && sh.e == e)
+    {
+        tw = p->tw_single.as<cfloat>();
+    }
+    else
+    {
+        std::lock_guard<std::mutex> lock(p->ref_mutex);
+        if (p->tw_em.ptr == nullptr)
+        {
+            const uint64_t entries = crdfft::stage_twiddle_entries(sh.complex_n, e);
+            CRDFFT_TRY(p->tw_em.alloc(std::max<uint64_t>(entries, 1) * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw_em.as<cfloat>(), sh.complex_n, e, s));
+        }
+        tw = p->tw_em.as<cfloat>();
+    }
+    crdfft::ColArgs a{static_cast<const cfloat*>(data), static_cast<cfloat*>(data), tw, nullptr, nullptr, 0, batch, 1, 0, 0};
+    CRDFFT_TRY(crdfft::launch_cols(sh.complex_n, e, c, inverse, a, s));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_execute_reference(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream)
+{
+    if (p == nullptr || in == nullptr || out == nullptr || in == out ||
+        (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    const crdfft::PlanShape& sh = p->shape;
+    if (sh.type != CRDFFT_C2C)
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t cn = sh.complex_n;
+    {
+        std::lock_guard<std::mutex> lock(p->ref_mutex);
+        if (p->wref.ptr == nullptr && cn >= 2)
+        {
+            CRDFFT_TRY(p->wref.alloc((cn / 2) * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_roots(p->wref.as<cfloat>(), cn / 2, 1, cn, s));
+        }
+    }
+    CRDFFT_TRY(crdfft::launch_reference(static_cast<const cfloat*>(in), static_cast<cfloat*>(out),
+                                        p->wref.as<cfloat>(), cn, sh.batch, p->idist, p->odist,
+                                        direction == CRDFFT_INVERSE, s));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
+{
+    if (p == nullptr || info == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    crdfft::shape_to_info(p->shape, info);
+    info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
+                        p->wreal.bytes;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_plan_describe(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
+                                   crdfft_plan_info* info)
+{
+    if (info == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    crdfft::PlanShape shape;
+    const crdfft_status st = crdfft::choose_shape(n, batch, type, precision, nullptr, &shape);
+    if (st != CRDFFT_OK)
+    {
+        return st;
+    }
+    crdfft::shape_to_info(shape, info);
+    return CRDFFT_OK;
+}
+
+const char* crdfft_status_string(crdfft_status status)
+{
+    switch (status)
+    {
+        case CRDFFT_OK:
+            return "ok";
+        case CRDFFT_INVALID_SIZE:
+            return "invalid size";
+        case CRDFFT_INVALID_ARGUMENT:
+            return "invalid argument";
+        case CRDFFT_OUT_OF_MEMORY:
+            return "out of device memory";
+        case CRDFFT_HIP_ERROR:
+            return "HIP runtime error";
+        case CRDFFT_SELF_CHECK_FAILED:
+            return "self-check failed";
+        case CRDFFT_UNSUPPORTED:
+            return "unsupported layout or precision";
+        case CRDFFT_NO_DEVICE:
+            return "no HIP device";
+    }
+    return "unknown status";
+}
+
+int32_t crdfft_last_hip_error(void) { return g_last_hip_error; }
+
+const char* crdfft_version(void) { return "crdfft 0.1 gfx950"; }
+
+// ---------------------------------------------------------------------------------------------------------------
+// runtime plumbing
+// ---------------------------------------------------------------------------------------------------------------
+crdfft_status crdfft_device_count(int32_t* count)
+{
+    if (count == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    int c = 0;
+    const hipError_t e = hipGetDeviceCount(&c);
+    *count = e == hipSuccess ? c : 0;
+    return e == hipSuccess ? CRDFFT_OK : CRDFFT_NO_DEVICE;
+}
+
+crdfft_status crdfft_device_props_get(int32_t device, crdfft_device_props* props)
+{
+    if (props == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    hipDeviceProp_t d{};
+    CRDFFT_TRY(hipGetDeviceProperties(&d, device));
+    std::memset(props, 0, sizeof(*props));
+    std::strncpy(props->name, d.name, sizeof(props->name) - 1);
+    std::strncpy(props->arch, d.gcnArchName, sizeof(props->arch) - 1);
+    props->compute_units = d.multiProcessorCount;
+    props->clock_khz = d.clockRate;
+    props->memory_clock_khz = d.memoryClockRate;
+    props->memory_bus_width = d.memoryBusWidth;
+    props->global_mem_bytes = d.totalGlobalMem;
+    props->l2_bytes = static_cast<uint64_t>(d.l2CacheSize);
+    props->max_lds_per_block = static_cast<int32_t>(d.sharedMemPerBlock);
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_malloc(void** ptr, uint64_t bytes)
+{
+    if (ptr == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *ptr = nullptr;
+    CRDFFT_TRY(hipMalloc(ptr, bytes));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_free(void* ptr)
+{
+    CRDFFT_TRY(hipFree(ptr));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_memcpy_h2d(void* dst, const void* src, uint64_t bytes)
+{
+    CRDFFT_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
+{
+    CRDFFT_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    CRDFFT_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_fill_bytes(void* dst, int32_t value, uint64_t bytes, void* stream)
+{
+    if (dst == nullptr && bytes != 0)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    CRDFFT_TRY(crdfft::launch_fill_bytes(dst, value, bytes, static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_stream_create(void** stream)
+{
+    if (stream == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    hipStream_t s = nullptr;
+    CRDFFT_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_stream_destroy(void* stream)
+{
+    CRDFFT_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_stream_synchronize(void* stream)
+{
+    CRDFFT_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_device_synchronize(void)
+{
+    CRDFFT_TRY(hipDeviceSynchronize());
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_event_create(void** event)
+{
+    if (event == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    hipEvent_t e = nullptr;
+    CRDFFT_TRY(hipEventCreate(&e));
+    *event = e;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_event_destroy(void* event)
+{
+    CRDFFT_TRY(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_event_record(void* event, void* stream)
+{
+    CRDFFT_TRY(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_event_synchronize(void* event)
+{
+    CRDFFT_TRY(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_event_elapsed_ms(float* ms, void* start, void* stop)
+{
+    if (ms == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    CRDFFT_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return CRDFFT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,212 @@
+// The planner (planner.hpp): host-only choice of kernels, tiles and chunking.
+#include "planner.hpp"
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace crdfft
+{
+
+bool is_pow2(uint64_t n) { return n != 0 && (n & (n - 1)) == 0; }
+
+uint32_t log2_exact(uint64_t n)
+{
+    uint32_t l = 0;
+    while ((uint64_t{1} << l) < n)
+    {
+        ++l;
+    }
+    return l;
+}
+
+bool have_row_kernel(uint64_t n, int e)
+{
+    LaunchShape s;
+    return row_kernel_shape(n, e, &s);
+}
+
+bool have_col_kernel(uint64_t n1, int e, int c)
+{
+    LaunchShape s;
+    return col_kernel_shape(n1, e, c, &s);
+}
+
+bool have_rowtr_kernel(uint64_t n2, int e, int r)
+{
+    LaunchShape s;
+    return rowtr_kernel_shape(n2, e, r, &s);
+}
+
+namespace
+{
+
+// Values per thread of the single kernel: every radix-16 stage but one (the first) costs one LDS round trip, and 16
+// values keep the data at 32 VGPRs; below 32 points one thread owns a whole transform.
+int default_e(uint64_t n) { return n <= 16 ? static_cast<int>(n) : 16; }
+
+// Four-step split of n = 2^L: n1 = 2^floor(L/2) columns' length, n2 = n / n1 rows' length (n2 >= n1, the row pass
+// carries the transposed store and gets the longer rows), tiles sized so one workgroup's LDS holds at most 16K
+// points.
+void default_four_step(uint64_t n, PlanShape* s)
+{
+    const uint32_t lg = log2_exact(n);
+    s->n1 = uint64_t{1} << (lg / 2);
+    s->n2 = n / s->n1;
+    s->e1 = 16;
+    s->e2 = 16;
+    s->tile1 = s->n1 <= 512 ? 16 : (s->n1 <= 2048 ? 8 : 4);
+    s->tile2 = s->n2 <= 512 ? 16 : (s->n2 <= 2048 ? 8 : 4);
+}
+
+} // namespace
+
+bool element_major_shape(uint64_t n, uint64_t batch, int* e, int* c)
+{
+    if (n <= 16)
+    {
+        *e = static_cast<int>(n);
+        *c = 64;
+    }
+    else if (n <= 64)
+    {
+        *e = 16;
+        *c = 16;
+    }
+    else if (n <= 1024)
+    {
+        *e = 16;
+        *c = batch >= 16 ? 16 : 8;
+        if (!have_col_kernel(n, *e, *c))
+        {
+            *c = 16;
+        }
+    }
+    else if (n <= 4096)
+    {
+        *e = 16;
+        *c = n == 4096 ? 4 : 8;
+    }
+    else
+    {
+        return false;
+    }
+    return have_col_kernel(n, *e, *c);
+}
+
+crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t precision, const crdfft_tuning* tuning,
+                           PlanShape* out)
+{
+    if (out == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    if (type != CRDFFT_C2C && type != CRDFFT_R2C && type != CRDFFT_C2R)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    if (precision != CRDFFT_FP32 && precision != CRDFFT_SPLIT_BF16 && precision != CRDFFT_SPLIT_FP16)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    if (!is_pow2(n))
+    {
+        return CRDFFT_INVALID_SIZE;
+    }
+    if (type != CRDFFT_C2C && n < 4)
+    {
+        return CRDFFT_INVALID_SIZE; // RealFftPlan: a power of two >= 4 (real_fft.hpp:24)
+    }
+    PlanShape s;
+    s.n = n;
+    s.batch = batch;
+    s.type = type;
+    s.precision = precision;
+    s.complex_n = type == CRDFFT_C2C ? n : n / 2;
+    const uint64_t cn = s.complex_n;
+    if (cn > kFourStepMax)
+    {
+        return CRDFFT_INVALID_SIZE;
+    }
+    if (cn == 1)
+    {
+        s.algorithm = CRDFFT_ALGO_COPY;
+    }
+    else if (cn <= kSingleMax)
+    {
+        s.algorithm = CRDFFT_ALGO_SINGLE;
+        s.e = default_e(cn);
+        if (tuning != nullptr && tuning->e != 0)
+        {
+            s.e = tuning->e;
+        }
+        if (!have_row_kernel(cn, s.e))
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+    }
+    else
+    {
+        s.algorithm = CRDFFT_ALGO_FOUR_STEP;
+        default_four_step(cn, &s);
+        if (tuning != nullptr)
+        {
+            if (tuning->n1 != 0)
+            {
+                if (!is_pow2(tuning->n1) || tuning->n1 >= cn)
+                {
+                    return CRDFFT_INVALID_ARGUMENT;
+                }
+                s.n1 = tuning->n1;
+                s.n2 = cn / s.n1;
+                s.tile1 = s.n1 <= 512 ? 16 : (s.n1 <= 2048 ? 8 : 4);
+                s.tile2 = s.n2 <= 512 ? 16 : (s.n2 <= 2048 ? 8 : 4);
+            }
+            if (tuning->e1 != 0)
+                s.e1 = tuning->e1;
+            if (tuning->tile1 != 0)
+                s.tile1 = tuning->tile1;
+            if (tuning->e2 != 0)
+                s.e2 = tuning->e2;
+            if (tuning->tile2 != 0)
+                s.tile2 = tuning->tile2;
+        }
+        if (!have_col_kernel(s.n1, s.e1, s.tile1) || !have_rowtr_kernel(s.n2, s.e2, s.tile2) ||
+            (s.n1 % static_cast<uint64_t>(s.tile2)) != 0)
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+        const uint64_t per = cn * 8;
+        s.chunk = std::max<uint64_t>(1, kChunkBytes / per);
+        if (tuning != nullptr && tuning->chunk != 0)
+        {
+            s.chunk = tuning->chunk;
+        }
+        s.chunk = std::min<uint64_t>(s.chunk, std::max<uint64_t>(batch, 1));
+        s.workspace_bytes = s.chunk * per;
+    }
+    *out = s;
+    return CRDFFT_OK;
+}
+
+void shape_to_info(const PlanShape& s, crdfft_plan_info* info)
+{
+    *info = crdfft_plan_info{};
+    info->n = s.n;
+    info->batch = s.batch;
+    info->type = s.type;
+    info->precision = s.precision;
+    info->algorithm = s.algorithm;
+    info->complex_n = s.complex_n;
+    info->n1 = s.n1;
+    info->n2 = s.n2;
+    info->e = s.e;
+    info->e1 = s.e1;
+    info->tile1 = s.tile1;
+    info->e2 = s.e2;
+    info->tile2 = s.tile2;
+    info->chunk = s.chunk;
+    info->workspace_bytes = s.workspace_bytes;
+}
+
+} // namespace crdfft

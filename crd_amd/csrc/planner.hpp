@@ -1,0 +1,56 @@
+// The planner: a pure function of (n, batch, type, precision, tuning) to the kernels a plan runs. Host-only, no
+// device calls, so crdfft_plan_describe can answer without a GPU.
+//
+// It plays the part of the reference's planned_forward / four_step_split / level_forward (engine/numerics/
+// hesap-fft/include/crd/hesap/fft/fft.hpp:63-210): there nine CPU algorithms are picked by size and CPU level;
+// here the choice is between one LDS-resident kernel (n <= 16384) and a two-kernel four-step (n1 x n2, each side
+// LDS-resident), with the values per thread, the tile widths and the chunk of transforms per pass pair as knobs.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/crdfft.h"
+
+namespace crdfft
+{
+
+struct PlanShape
+{
+    uint64_t n = 0;         // user length (real length for R2C/C2R)
+    uint64_t batch = 0;
+    int32_t type = CRDFFT_C2C;
+    int32_t precision = CRDFFT_FP32;
+    int32_t algorithm = CRDFFT_ALGO_COPY;
+    uint64_t complex_n = 0;
+    uint64_t n1 = 0, n2 = 0;
+    int32_t e = 0;
+    int32_t e1 = 0, tile1 = 0;
+    int32_t e2 = 0, tile2 = 0;
+    uint64_t chunk = 0;
+    uint64_t workspace_bytes = 0;
+};
+
+// The largest complex length one LDS-resident kernel runs.
+constexpr uint64_t kSingleMax = 16384;
+// The largest complex length of the plain path (two LDS-resident passes of at most 4096 each).
+constexpr uint64_t kFourStepMax = uint64_t{1} << 24;
+// Bytes of four-step intermediate kept in flight between the column and the row pass: small enough that the
+// intermediate stays in the 256 MB Infinity Cache, large enough that each pass fills the GPU.
+constexpr uint64_t kChunkBytes = uint64_t{128} << 20;
+
+bool is_pow2(uint64_t n);
+uint32_t log2_exact(uint64_t n);
+
+crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t precision, const crdfft_tuning* tuning,
+                           PlanShape* out);
+void shape_to_info(const PlanShape& s, crdfft_plan_info* info);
+
+// Values per thread and columns per workgroup of the element-major kernel for n (e depends on n only).
+bool element_major_shape(uint64_t n, uint64_t batch, int* e, int* c);
+
+// The instantiated kernel shapes (mirrors of the kernels' tables, for the planner and the autotuner).
+bool have_row_kernel(uint64_t n, int e);
+bool have_col_kernel(uint64_t n1, int e, int c);
+bool have_rowtr_kernel(uint64_t n2, int e, int r);
+
+} // namespace crdfft
