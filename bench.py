@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Batched FP32 C2C FFT throughput on one MI355X (gfx950), through the crdfft C-ABI.
+
+Workload: the bandwidth-roofline configuration of the reference's batched FFT path, plain FP32 1D complex-to-complex
+forward transforms, unnormalized, interleaved complex, contiguous: n = 2^16 x 4096 transforms (4 GiB of algorithmic
+traffic) and n = 2^20 x 64 transforms (1 GiB). Inputs are i.i.d. standard normal (seeded), resident in HBM before the
+timed region. One step = one forward transform of each shape. value = algorithmic bytes (16 per complex point:
+one read, one write) of a step / step time, in GB/s; the percentage of the 8.0 TB/s HBM peak is reported beside it.
+
+Also reported (not the value): the R2C -> C2R round trip of both shapes, n = 4096 x 1024 (rotating over 4 buffers so
+the 64 MiB set cannot sit in the 256 MB Infinity Cache), the accuracy of a sample against FP64, a device-copy
+ceiling, the per-kernel roofline of the dominant kernel timed live with hipEvents around each launch, and the CPU
+baseline (the oracle's FP32 restatement of the reference's CPU algorithm, one plan per thread over contiguous ranges,
+timed on a bounded sample on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "batched C2C GB/s (% HBM peak); tuned vs plain FP32 FFTs/s at rel-L2≤1e-6"
+HBM_PEAK_GBS = 8000.0
+SHAPES = [(1 << 16, 4096), (1 << 20, 64)]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def fill_normal(buf, count_floats, seed, chunk=1 << 24):
+    """Upload count_floats i.i.d. N(0,1) float32 values (seeded) to buf; returns the first chunk (host copy)."""
+    rng = np.random.default_rng(seed)
+    first = None
+    done = 0
+    while done < count_floats:
+        m = min(chunk, count_floats - done)
+        a = rng.standard_normal(m, dtype=np.float32)
+        if first is None:
+            first = a.copy()
+        buf.upload(a, offset=done * 4)
+        done += m
+    return first
+
+
+def time_loop(fn, stream, steps, warmup, Event):
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def rel_l2(y, ref):
+    return float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
+
+
+def cpu_baseline(seconds_per_shape, threads):
+    """The oracle's FP32 restatement on a bounded sample of the same shapes, on `threads` host threads."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    from oracle import pyoracle as po
+
+    total_bytes = 0.0
+    total_s = 0.0
+    parts = []
+    for n, batch in SHAPES:
+        # a sample of whole transforms: grow until one pass takes >= seconds_per_shape
+        count = max(threads, 1)
+        x = np.random.default_rng(7).standard_normal(2 * n * count, dtype=np.float32).view(np.complex64)
+        x = x.reshape(count, n)
+        while True:
+            y = x.copy()
+            t0 = time.perf_counter()
+            po.c2c32_inplace(y, inverse=False, threads=threads)
+            dt = time.perf_counter() - t0
+            if dt >= seconds_per_shape or count >= batch:
+                break
+            grow = min(batch, max(count * 2, int(count * seconds_per_shape / max(dt, 1e-3))))
+            if grow == count:
+                break
+            count = grow
+            x = np.random.default_rng(7).standard_normal(2 * n * count, dtype=np.float32).view(np.complex64)
+            x = x.reshape(count, n)
+        b = 16.0 * n * count
+        parts.append(f"{count} x n={n} in {dt:.2f} s")
+        # scale each shape to its share of one step (batch transforms), so the CPU figure is the same step mix
+        total_bytes += 16.0 * n * batch
+        total_s += dt * batch / count
+    return {
+        "value": total_bytes / total_s / 1e9,
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "FP32 Stockham/four-step restatement (oracle/), one plan per thread over contiguous ranges: "
+                  + "; ".join(parts) + "; scaled to one step of the GPU workload",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="only the value (for profiler runs)")
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the batched FFT shards by splitting "
+              "the batch, with no exchange, so the one-GPU figure is the per-GPU share", file=sys.stderr)
+        sys.exit(2)
+
+    import crd_amd
+    from crd_amd import DeviceBuffer, Event, FftPlan, Stream
+
+    if crd_amd.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    props = crd_amd.device_props(0)
+    log(f"device {props['arch']} CUs={props['compute_units']}")
+    stream = Stream()
+
+    plans, bufs, firsts = [], [], []
+    for idx, (n, batch) in enumerate(SHAPES):
+        log(f"plan c2c n={n} batch={batch}")
+        p = FftPlan(n, batch)
+        nbytes = 8 * n * batch
+        xin = DeviceBuffer(nbytes)
+        xout = DeviceBuffer(nbytes)
+        firsts.append(fill_normal(xin, 2 * n * batch, seed=1000 + idx))
+        plans.append(p)
+        bufs.append((xin, xout))
+
+    def step():
+        for p, (xin, xout) in zip(plans, bufs):
+            p.execute(xin, xout, crd_amd.FORWARD, stream)
+
+    log(f"timing {args.steps} steps after {args.warmup} warmup")
+    ms_step = time_loop(step, stream, args.steps, args.warmup, Event)
+    step_bytes = sum(16.0 * n * b for n, b in SHAPES)
+    value = step_bytes / (ms_step * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (i.i.d. N(0,1) from a fixed seed, resident in HBM)",
+        "config": {
+            "workload": "plain FP32 1D C2C forward, interleaved, contiguous, unnormalized: "
+                        "n=65536 x batch 4096 + n=1048576 x batch 64 per step",
+            "shapes": [{"n": n, "batch": b} for n, b in SHAPES],
+            "parallelism": "single GPU",
+        },
+        "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
+    }
+
+    # per-shape figures and the live per-kernel roofline
+    per_shape = []
+    best = None
+    for (n, batch), p, (xin, xout) in zip(SHAPES, plans, bufs):
+        ms = time_loop(lambda: p.execute(xin, xout, crd_amd.FORWARD, stream), stream, max(3, args.steps),
+                       1, Event)
+        gbs = 16.0 * n * batch / (ms * 1e-3) / 1e9
+        acc = None
+        for _ in range(3):
+            acc = p.execute_profiled(xin, xout, crd_amd.FORWARD, stream, acc)
+        info = p.info()
+        names = ["single" if info["algorithm"] == 1 else "column pass", "row pass (transposed store)", "real", "copy"]
+        kernels = []
+        for ph in range(4):
+            if acc.launches[ph] == 0:
+                continue
+            avg_ms = acc.ms[ph] / acc.launches[ph]
+            pts = acc.points[ph] / acc.launches[ph]
+            ach = 16.0 * pts / (avg_ms * 1e-3) / 1e9
+            k = {"phase": names[ph], "launches_per_call": acc.launches[ph] // 3, "avg_ms": round(avg_ms, 5),
+                 "points_per_launch": int(pts), "achieved_gbs": round(ach, 1), "total_ms": acc.ms[ph] / 3}
+            kernels.append(k)
+            if best is None or k["total_ms"] > best[0]["total_ms"]:
+                best = (k, n, batch, info)
+        per_shape.append({"n": n, "batch": batch, "ms": round(ms, 4), "gbs": round(gbs, 1),
+                          "ffts_per_s": round(batch / (ms * 1e-3), 1), "plan": {k: info[k] for k in (
+                              "algorithm", "n1", "n2", "e", "e1", "tile1", "e2", "tile2", "chunk")},
+                          "kernels": kernels})
+    result["per_shape"] = per_shape
+    k, n, batch, info = best
+    result["roofline"] = {
+        "bound": "hbm",
+        "achieved": k["achieved_gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": f"{k['phase']} of n={n} (algorithmic 16 B per complex point per launch; "
+                  f"{k['points_per_launch']} points per launch, avg {k['avg_ms']} ms, hipEvents on the launch stream)",
+    }
+    traffic_file = os.path.join(HERE, "profiles", "hbm_traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file))
+            key = f"n={n} {k['phase']}"
+            if key in tr:
+                result["roofline"]["traffic"] = tr[key]["bytes_per_launch"]
+        except (ValueError, KeyError):
+            pass
+
+    # accuracy of a sample vs FP64 (numpy's pocketfft in float64)
+    acc_report = []
+    for (n, batch), (xin, xout), first in zip(SHAPES, bufs, firsts):
+        m = min(len(first) // (2 * n), 4)
+        x = first[: 2 * n * m].view(np.complex64).reshape(m, n)
+        y = xout.download((m, n), np.complex64)
+        ref = np.fft.fft(x.astype(np.complex128), axis=1)
+        acc_report.append({"n": n, "transforms": m, "rel_l2_vs_fp64": rel_l2(y.astype(np.complex128), ref)})
+    result["accuracy"] = acc_report
+
+    if not args.no_extras:
+        extras = {}
+        # device copy ceiling (read + write bytes)
+        xin, xout = bufs[0]
+        ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
+        extras["device_copy_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
+        # R2C -> C2R round trip of both shapes
+        rts = []
+        for idx, (n, batch) in enumerate(SHAPES):
+            r2c = FftPlan(n, batch, "r2c")
+            c2r = FftPlan(n, batch, "c2r")
+            real_in = DeviceBuffer(4 * n * batch)
+            spec = DeviceBuffer(8 * (n // 2 + 1) * batch)
+            back = DeviceBuffer(4 * n * batch)
+            first = fill_normal(real_in, n * batch, seed=2000 + idx)
+
+            def rt():
+                r2c.execute(real_in, spec, crd_amd.FORWARD, stream)
+                c2r.execute(spec, back, crd_amd.INVERSE, stream)
+
+            ms = time_loop(rt, stream, max(3, args.steps // 2), 1, Event)
+            bytes_rt = float(batch) * (4 * n + 8 * (n // 2 + 1)) * 2
+            m = min(4, batch)
+            xb = back.download((m, n), np.float32)
+            err = float(np.max(np.abs(xb - first[: m * n].reshape(m, n))))
+            rts.append({"n": n, "batch": batch, "ms": round(ms, 4), "gbs": round(bytes_rt / (ms * 1e-3) / 1e9, 1),
+                        "max_abs_roundtrip_err": err})
+            del r2c, c2r, real_in, spec, back
+        extras["r2c_c2r_roundtrip"] = rts
+        # n = 4096 x 1024 over 4 rotating buffers
+        n, batch, nb = 4096, 1024, 4
+        p = FftPlan(n, batch)
+        ins = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
+        outs = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
+        for i, b in enumerate(ins):
+            fill_normal(b, 2 * n * batch, seed=3000 + i)
+        state = {"i": 0}
+
+        def rot():
+            i = state["i"] % nb
+            state["i"] += 1
+            p.execute(ins[i], outs[i], crd_amd.FORWARD, stream)
+
+        ms = time_loop(rot, stream, 40, 4, Event)
+        extras["c2c_n4096_b1024"] = {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
+                                     "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
+        result["extras"] = extras
+
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds, threads)
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,14 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class PhaseTimes(ctypes.Structure):
+    _fields_ = [
+        ("ms", ctypes.c_double * 4),
+        ("launches", ctypes.c_uint64 * 4),
+        ("points", ctypes.c_uint64 * 4),
+    ]
+
+
 class DeviceProps(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 256),
@@ -94,6 +102,7 @@ _SIGS = {
     "crdfft_execute_batched": (_i32, [_vp, _vp, _u64, _i32, _vp]),
     "crdfft_execute_reference": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_self_check": (_i32, [_vp]),
+    "crdfft_execute_profiled": (_i32, [_vp, _vp, _vp, _i32, _vp, ctypes.POINTER(PhaseTimes)]),
     "crdfft_plan_get_info": (_i32, [_vp, ctypes.POINTER(PlanInfo)]),
     "crdfft_plan_describe": (_i32, [_u64, _u64, _i32, _i32, ctypes.POINTER(PlanInfo)]),
     "crdfft_status_string": (ctypes.c_char_p, [_i32]),

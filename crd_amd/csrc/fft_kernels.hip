@@ -95,7 +95,7 @@ __global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {
-            dst[i + C::T * m] = v[m];
+            dst[i + C::T * m] = cf{v[m].x * a.scale, v[m].y * a.scale};
         }
     }
 }
@@ -244,7 +244,7 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     for (int m = 0; m < E; ++m)
     {
         const int k2 = j2 + C::T * m;
-        dst[static_cast<uint64_t>(k2) * a.n1] = cf{tre[k2 * C::TP + r2], tim[k2 * C::TP + r2]};
+        dst[static_cast<uint64_t>(k2) * a.n1] = cf{tre[k2 * C::TP + r2] * a.scale, tim[k2 * C::TP + r2] * a.scale};
     }
 }
 

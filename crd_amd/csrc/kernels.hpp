@@ -24,6 +24,7 @@ struct RowArgs
     uint64_t count;
     uint64_t idist;
     uint64_t odist;
+    float scale; // applied to every output (1 = unscaled, exact)
 };
 
 // Column transforms of length n1 over `ncols` adjacent columns of a row-major n1 x ncols matrix, for `count`
@@ -54,6 +55,7 @@ struct RowTrArgs
     uint64_t count;
     uint64_t idist;
     uint64_t odist;
+    float scale; // applied to every output (1 = unscaled, exact)
 };
 
 struct LaunchShape

@@ -84,6 +84,7 @@ struct crdfft_plan_s
     crdfft::PlanShape shape;
     uint64_t idist = 0;
     uint64_t odist = 0;
+    uint32_t flags = 0;
     int device = 0;
     // twiddle tables
     DeviceBuffer tw_single; // stage table of the single kernel (complex_n)
@@ -99,6 +100,75 @@ struct crdfft_plan_s
 
 namespace
 {
+
+// Event pairs around launches for crdfft_execute_profiled; a null profiler records nothing.
+struct Profiler
+{
+    crdfft_phase_times* acc = nullptr;
+    hipStream_t stream = nullptr;
+    struct Mark
+    {
+        hipEvent_t a, b;
+        int phase;
+        uint64_t points;
+    };
+    std::vector<Mark> marks;
+    hipEvent_t open_event = nullptr;
+
+    hipError_t before()
+    {
+        if (acc == nullptr)
+        {
+            return hipSuccess;
+        }
+        hipError_t e = hipEventCreate(&open_event);
+        if (e != hipSuccess)
+        {
+            return e;
+        }
+        return hipEventRecord(open_event, stream);
+    }
+    hipError_t after(int phase, uint64_t points)
+    {
+        if (acc == nullptr)
+        {
+            return hipSuccess;
+        }
+        hipEvent_t b = nullptr;
+        hipError_t e = hipEventCreate(&b);
+        if (e != hipSuccess)
+        {
+            return e;
+        }
+        e = hipEventRecord(b, stream);
+        marks.push_back({open_event, b, phase, points});
+        open_event = nullptr;
+        return e;
+    }
+    hipError_t finish()
+    {
+        if (acc == nullptr)
+        {
+            return hipSuccess;
+        }
+        hipError_t err = hipStreamSynchronize(stream);
+        for (const Mark& m : marks)
+        {
+            float ms = 0.0f;
+            if (err == hipSuccess)
+            {
+                err = hipEventElapsedTime(&ms, m.a, m.b);
+            }
+            acc->ms[m.phase] += ms;
+            acc->launches[m.phase] += 1;
+            acc->points[m.phase] += m.points;
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+        }
+        marks.clear();
+        return err;
+    }
+};
 
 crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
 {
@@ -139,7 +209,7 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
 
 // The complex transform of the plan over `count` transforms (complex distances), on `s`.
 crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint64_t count, uint64_t idist,
-                          uint64_t odist, bool inverse, hipStream_t s)
+                          uint64_t odist, bool inverse, hipStream_t s, Profiler& prof, float scale = 1.0f)
 {
     const crdfft::PlanShape& sh = p->shape;
     const uint64_t cn = sh.complex_n;
@@ -148,13 +218,17 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
         case CRDFFT_ALGO_COPY:
             if (in != out)
             {
+                CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_copy(in, out, cn, count, idist, odist, s));
+                CRDFFT_TRY(prof.after(3, cn * count));
             }
             return CRDFFT_OK;
         case CRDFFT_ALGO_SINGLE:
         {
-            crdfft::RowArgs a{in, out, p->tw_single.as<cfloat>(), count, idist, odist};
+            crdfft::RowArgs a{in, out, p->tw_single.as<cfloat>(), count, idist, odist, scale};
+            CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_rows(cn, sh.e, inverse, a, s));
+            CRDFFT_TRY(prof.after(0, cn * count));
             return CRDFFT_OK;
         }
         case CRDFFT_ALGO_FOUR_STEP:
@@ -172,15 +246,86 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                    cnt,
                                    idist,
                                    cn};
+                CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
+                CRDFFT_TRY(prof.after(0, cn * cnt));
                 crdfft::RowTrArgs ra{p->workspace.as<cfloat>(), out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
-                                     cn, odist};
+                                     cn, odist, scale};
+                CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
+                CRDFFT_TRY(prof.after(1, cn * cnt));
             }
             return CRDFFT_OK;
         }
         default:
             return CRDFFT_UNSUPPORTED;
+    }
+}
+
+crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream, Profiler& prof)
+{
+    if (p == nullptr || in == nullptr || out == nullptr ||
+        (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    const crdfft::PlanShape& sh = p->shape;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool inverse = direction == CRDFFT_INVERSE;
+    if (sh.batch == 0)
+    {
+        return CRDFFT_OK;
+    }
+    if (sh.precision != CRDFFT_FP32)
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    switch (sh.type)
+    {
+        case CRDFFT_C2C:
+        {
+            // CRDFFT_FLAG_NORMALIZE_INVERSE: ifft_normalized's 1/n (fft.hpp:1015-1029), fused into the last store
+            const float scale =
+                (inverse && (p->flags & CRDFFT_FLAG_NORMALIZE_INVERSE) != 0) ? 1.0f / static_cast<float>(sh.n) : 1.0f;
+            return run_complex(p, static_cast<const cfloat*>(in), static_cast<cfloat*>(out), sh.batch, p->idist,
+                               p->odist, inverse, s, prof, scale);
+        }
+        case CRDFFT_R2C:
+        {
+            // rfft is a forward transform: the direction argument must say so
+            if (inverse || in == out)
+            {
+                return CRDFFT_INVALID_ARGUMENT;
+            }
+            const uint64_t h = sh.n / 2;
+            cfloat* o = static_cast<cfloat*>(out);
+            const crdfft_status st =
+                run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist / 2, p->odist, false, s, prof);
+            if (st != CRDFFT_OK)
+            {
+                return st;
+            }
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_r2c_post(o, p->wreal.as<cfloat>(), h, sh.batch, p->odist, s));
+            CRDFFT_TRY(prof.after(2, (h + 1) * sh.batch));
+            return CRDFFT_OK;
+        }
+        case CRDFFT_C2R:
+        {
+            if (!inverse || in == out)
+            {
+                return CRDFFT_INVALID_ARGUMENT;
+            }
+            const uint64_t h = sh.n / 2;
+            cfloat* o = static_cast<cfloat*>(out);
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_c2r_pre(static_cast<const cfloat*>(in), o, p->wreal.as<cfloat>(), h, sh.batch,
+                                              p->idist, p->odist / 2, 1.0f / static_cast<float>(h), s));
+            CRDFFT_TRY(prof.after(2, (h + 1) * sh.batch));
+            return run_complex(p, o, o, sh.batch, p->odist / 2, p->odist / 2, true, s, prof);
+        }
+        default:
+            return CRDFFT_INVALID_ARGUMENT;
     }
 }
 
@@ -230,6 +375,7 @@ crdfft_status create_plan(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_
     p->shape = shape;
     p->idist = idist;
     p->odist = odist;
+    p->flags = flags;
     (void)hipGetDevice(&p->device);
     crdfft_status rs = build_tables(p, nullptr);
     if (rs == CRDFFT_OK)
@@ -278,60 +424,27 @@ crdfft_status crdfft_plan_destroy(crdfft_plan plan)
 
 crdfft_status crdfft_execute(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream)
 {
-    if (p == nullptr || in == nullptr || out == nullptr ||
-        (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
+    Profiler none;
+    return execute_impl(p, in, out, direction, stream, none);
+}
+
+crdfft_status crdfft_execute_profiled(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream,
+                                      crdfft_phase_times* acc)
+{
+    if (acc == nullptr)
     {
         return CRDFFT_INVALID_ARGUMENT;
     }
-    const crdfft::PlanShape& sh = p->shape;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool inverse = direction == CRDFFT_INVERSE;
-    if (sh.batch == 0)
+    Profiler prof;
+    prof.acc = acc;
+    prof.stream = static_cast<hipStream_t>(stream);
+    const crdfft_status st = execute_impl(p, in, out, direction, stream, prof);
+    const hipError_t e = prof.finish();
+    if (st != CRDFFT_OK)
     {
-        return CRDFFT_OK;
+        return st;
     }
-    if (sh.precision != CRDFFT_FP32)
-    {
-        return CRDFFT_UNSUPPORTED;
-    }
-    switch (sh.type)
-    {
-        case CRDFFT_C2C:
-            return run_complex(p, static_cast<const cfloat*>(in), static_cast<cfloat*>(out), sh.batch, p->idist,
-                               p->odist, inverse, s);
-        case CRDFFT_R2C:
-        {
-            // rfft is a forward transform: the direction argument must say so
-            if (inverse || in == out)
-            {
-                return CRDFFT_INVALID_ARGUMENT;
-            }
-            const uint64_t h = sh.n / 2;
-            cfloat* o = static_cast<cfloat*>(out);
-            const crdfft_status st =
-                run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist / 2, p->odist, false, s);
-            if (st != CRDFFT_OK)
-            {
-                return st;
-            }
-            CRDFFT_TRY(crdfft::launch_r2c_post(o, p->wreal.as<cfloat>(), h, sh.batch, p->odist, s));
-            return CRDFFT_OK;
-        }
-        case CRDFFT_C2R:
-        {
-            if (!inverse || in == out)
-            {
-                return CRDFFT_INVALID_ARGUMENT;
-            }
-            const uint64_t h = sh.n / 2;
-            cfloat* o = static_cast<cfloat*>(out);
-            CRDFFT_TRY(crdfft::launch_c2r_pre(static_cast<const cfloat*>(in), o, p->wreal.as<cfloat>(), h, sh.batch,
-                                              p->idist, p->odist / 2, 1.0f / static_cast<float>(h), s));
-            return run_complex(p, o, o, sh.batch, p->odist / 2, p->odist / 2, true, s);
-        }
-        default:
-            return CRDFFT_INVALID_ARGUMENT;
-    }
+    return hip_status(e);
 }
 
 crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, int32_t direction, void* stream)

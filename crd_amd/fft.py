@@ -194,6 +194,14 @@ class FftPlan:
         check(N.lib().crdfft_execute(self.handle, _ptr(data_in), _ptr(out), int(direction), _stream(stream)),
               "crdfft_execute")
 
+    def execute_profiled(self, data_in, data_out=None, direction=N.FORWARD, stream=None, acc=None):
+        """execute() with hipEvents around every launch; returns the accumulated per-phase times."""
+        acc = N.PhaseTimes() if acc is None else acc
+        out = data_in if data_out is None else data_out
+        check(N.lib().crdfft_execute_profiled(self.handle, _ptr(data_in), _ptr(out), int(direction), _stream(stream),
+                                              ctypes.byref(acc)), "crdfft_execute_profiled")
+        return acc
+
     def execute_batched(self, data, batch, direction=N.FORWARD, stream=None):
         check(N.lib().crdfft_execute_batched(self.handle, _ptr(data), int(batch), int(direction), _stream(stream)),
               "crdfft_execute_batched")

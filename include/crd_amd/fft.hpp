@@ -1,0 +1,264 @@
+// C++20 host interface of the MI355X batched FFT path: the reference's FftPlan / RealFftPlan / fft /
+// ifft_normalized (engine/numerics/hesap-fft/include/crd/hesap/fft/fft.hpp:220-1029, real_fft.hpp:18-101) over the
+// crdfft C-ABI, header-only.
+//
+// Same names, argument meaning and conventions as the reference: forward and inverse unnormalized, irfft scaled by
+// 1/n, Complex<T> interleaved {re, im}, execute in place on n values, execute_batched element-major
+// (data[i*batch + t]). Differences, all forced by the device boundary:
+//   - data lives in device memory (DeviceSpan), and calls are asynchronous on the plan's HIP stream;
+//   - a plan covers `batch` contiguous transforms (the reference loops over transforms and jobs, batch_arms.hpp);
+//   - the allocator is the device's (the reference takes an IAllocator*; a device plan owns hipMalloc'd tables);
+//   - a contract violation throws FftError carrying the crdfft status (the reference asserts, fft.hpp:237).
+// Only T = float exists: the GPU path is FP32.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "../crdfft.h"
+#include "../crdfft_runtime.h"
+
+namespace crd_amd::hesap::fft
+{
+
+enum class FftDirection : std::uint8_t
+{
+    Forward = 0,
+    Inverse = 1,
+};
+
+// Layout of crd::hesap::Complex<T> (engine/numerics/hesap/include/crd/hesap/complex.hpp:27-35).
+template <typename T> struct Complex
+{
+    T re;
+    T im;
+};
+static_assert(sizeof(Complex<float>) == 8, "interleaved FP32 complex");
+
+class FftError : public std::runtime_error
+{
+public:
+    FftError(crdfft_status status, const char* what)
+        : std::runtime_error(std::string(what) + ": " + crdfft_status_string(status)), m_status(status)
+    {
+    }
+    [[nodiscard]] crdfft_status status() const noexcept { return m_status; }
+
+private:
+    crdfft_status m_status;
+};
+
+inline void check(crdfft_status st, const char* what)
+{
+    if (st != CRDFFT_OK)
+    {
+        throw FftError(st, what);
+    }
+}
+
+// A view of device memory: `size` values of T at `data`.
+template <typename T> class DeviceSpan
+{
+public:
+    constexpr DeviceSpan() = default;
+    constexpr DeviceSpan(T* data, std::size_t size) : m_data(data), m_size(size) {}
+    [[nodiscard]] constexpr T* data() const noexcept { return m_data; }
+    [[nodiscard]] constexpr std::size_t size() const noexcept { return m_size; }
+
+private:
+    T* m_data = nullptr;
+    std::size_t m_size = 0;
+};
+
+// An owning device allocation of `count` values of T.
+template <typename T> class DeviceArray
+{
+public:
+    explicit DeviceArray(std::size_t count) : m_size(count)
+    {
+        void* p = nullptr;
+        check(crdfft_malloc(&p, count * sizeof(T)), "crdfft_malloc");
+        m_data = static_cast<T*>(p);
+    }
+    ~DeviceArray()
+    {
+        if (m_data != nullptr)
+        {
+            (void)crdfft_free(m_data);
+        }
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : m_data(std::exchange(o.m_data, nullptr)), m_size(o.m_size) {}
+    DeviceArray& operator=(DeviceArray&&) = delete;
+
+    [[nodiscard]] DeviceSpan<T> span() const noexcept { return {m_data, m_size}; }
+    [[nodiscard]] T* data() const noexcept { return m_data; }
+    [[nodiscard]] std::size_t size() const noexcept { return m_size; }
+    void upload(const T* host) { check(crdfft_memcpy_h2d(m_data, host, m_size * sizeof(T)), "crdfft_memcpy_h2d"); }
+    void download(T* host) const
+    {
+        check(crdfft_memcpy_d2h(host, m_data, m_size * sizeof(T)), "crdfft_memcpy_d2h");
+    }
+
+private:
+    T* m_data = nullptr;
+    std::size_t m_size = 0;
+};
+
+template <typename T> class FftPlan
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+
+public:
+    // A plan of `batch` contiguous transforms of n (fft.hpp:224-227); `stream` is a hipStream_t or null.
+    explicit FftPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr) : m_n(n), m_batch(batch), m_stream(stream)
+    {
+        check(crdfft_plan_create(&m_plan, n, batch, 0, 0, CRDFFT_C2C, CRDFFT_FP32, CRDFFT_FLAG_NONE),
+              "crdfft_plan_create");
+    }
+    ~FftPlan()
+    {
+        if (m_plan != nullptr)
+        {
+            (void)crdfft_plan_destroy(m_plan);
+        }
+    }
+    // pinned like the reference's plan (fft.hpp:348-351)
+    FftPlan(const FftPlan&) = delete;
+    FftPlan& operator=(const FftPlan&) = delete;
+    FftPlan(FftPlan&&) = delete;
+    FftPlan& operator=(FftPlan&&) = delete;
+
+    [[nodiscard]] std::size_t size() const noexcept { return m_n; }
+    [[nodiscard]] std::size_t batch() const noexcept { return m_batch; }
+
+    // In place over the plan's batch (fft.hpp:409-413): data.size() == n * batch.
+    void execute(DeviceSpan<Complex<T>> data, FftDirection dir) const
+    {
+        require(data.size() == m_n * m_batch, "FftPlan::execute: size mismatch");
+        check(crdfft_execute(m_plan, data.data(), data.data(), static_cast<int32_t>(dir), m_stream), "crdfft_execute");
+    }
+    // Out of place.
+    void execute(DeviceSpan<const Complex<T>> in, DeviceSpan<Complex<T>> out, FftDirection dir) const
+    {
+        require(in.size() == m_n * m_batch && out.size() == m_n * m_batch, "FftPlan::execute: size mismatch");
+        check(crdfft_execute(m_plan, in.data(), out.data(), static_cast<int32_t>(dir), m_stream), "crdfft_execute");
+    }
+    // Element-major batch in place (fft.hpp:462-470): data.size() == n * batch.
+    void execute_batched(DeviceSpan<Complex<T>> data, std::size_t batch, FftDirection dir) const
+    {
+        require(data.size() == m_n * batch, "FftPlan::execute_batched: size mismatch");
+        check(crdfft_execute_batched(m_plan, data.data(), batch, static_cast<int32_t>(dir), m_stream),
+              "crdfft_execute_batched");
+    }
+    // The radix-2 reference transform (fft.hpp:415-460), out of place.
+    void execute_reference(DeviceSpan<const Complex<T>> in, DeviceSpan<Complex<T>> out, FftDirection dir) const
+    {
+        require(in.size() == m_n * m_batch && out.size() == m_n * m_batch, "FftPlan::execute_reference: size mismatch");
+        check(crdfft_execute_reference(m_plan, in.data(), out.data(), static_cast<int32_t>(dir), m_stream),
+              "crdfft_execute_reference");
+    }
+    [[nodiscard]] crdfft_plan_info state() const
+    {
+        crdfft_plan_info info{};
+        check(crdfft_plan_get_info(m_plan, &info), "crdfft_plan_get_info");
+        return info;
+    }
+    [[nodiscard]] crdfft_plan handle() const noexcept { return m_plan; }
+
+private:
+    static void require(bool ok, const char* what)
+    {
+        if (!ok)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, what);
+        }
+    }
+    std::size_t m_n;
+    std::size_t m_batch;
+    void* m_stream;
+    crdfft_plan m_plan = nullptr;
+};
+
+// real[n] <-> complex[n/2+1] over `batch` transforms (real_fft.hpp:18-101).
+template <typename T> class RealFftPlan
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+
+public:
+    explicit RealFftPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr) : m_n(n), m_batch(batch), m_stream(stream)
+    {
+        check(crdfft_plan_create(&m_r2c, n, batch, 0, 0, CRDFFT_R2C, CRDFFT_FP32, CRDFFT_FLAG_NONE),
+              "crdfft_plan_create(R2C)");
+        const crdfft_status st = crdfft_plan_create(&m_c2r, n, batch, 0, 0, CRDFFT_C2R, CRDFFT_FP32, CRDFFT_FLAG_NONE);
+        if (st != CRDFFT_OK)
+        {
+            (void)crdfft_plan_destroy(m_r2c);
+            throw FftError(st, "crdfft_plan_create(C2R)");
+        }
+    }
+    ~RealFftPlan()
+    {
+        (void)crdfft_plan_destroy(m_r2c);
+        (void)crdfft_plan_destroy(m_c2r);
+    }
+    RealFftPlan(const RealFftPlan&) = delete;
+    RealFftPlan& operator=(const RealFftPlan&) = delete;
+
+    [[nodiscard]] std::size_t size() const noexcept { return m_n; }
+
+    // real_fft.hpp:41-64: out[k], k <= n/2, per transform
+    void rfft(DeviceSpan<const T> real_in, DeviceSpan<Complex<T>> out) const
+    {
+        if (real_in.size() != m_n * m_batch || out.size() != (m_n / 2 + 1) * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "RealFftPlan::rfft: size mismatch");
+        }
+        check(crdfft_execute(m_r2c, real_in.data(), out.data(), CRDFFT_FORWARD, m_stream), "crdfft_execute(R2C)");
+    }
+    // real_fft.hpp:67-93: normalized, irfft(rfft(x)) == x
+    void irfft(DeviceSpan<const Complex<T>> in, DeviceSpan<T> real_out) const
+    {
+        if (in.size() != (m_n / 2 + 1) * m_batch || real_out.size() != m_n * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "RealFftPlan::irfft: size mismatch");
+        }
+        check(crdfft_execute(m_c2r, in.data(), real_out.data(), CRDFFT_INVERSE, m_stream), "crdfft_execute(C2R)");
+    }
+
+private:
+    std::size_t m_n;
+    std::size_t m_batch;
+    void* m_stream;
+    crdfft_plan m_r2c = nullptr;
+    crdfft_plan m_c2r = nullptr;
+};
+
+// One-shot transient plan, in place, unnormalized (fft.hpp:1005-1012). Synchronous.
+template <typename T> void fft(DeviceSpan<Complex<T>> data, FftDirection dir = FftDirection::Forward)
+{
+    const FftPlan<T> plan(data.size());
+    plan.execute(data, dir);
+    check(crdfft_device_synchronize(), "crdfft_device_synchronize");
+}
+
+// Normalized inverse, ifft_normalized(fft(x)) == x (fft.hpp:1015-1029): the 1/n rides on the last store.
+template <typename T> void ifft_normalized(DeviceSpan<Complex<T>> data)
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+    crdfft_plan plan = nullptr;
+    check(crdfft_plan_create(&plan, data.size(), 1, 0, 0, CRDFFT_C2C, CRDFFT_FP32, CRDFFT_FLAG_NORMALIZE_INVERSE),
+          "crdfft_plan_create");
+    const crdfft_status st = crdfft_execute(plan, data.data(), data.data(), CRDFFT_INVERSE, nullptr);
+    const crdfft_status sy = crdfft_device_synchronize();
+    (void)crdfft_plan_destroy(plan);
+    check(st, "crdfft_execute");
+    check(sy, "crdfft_device_synchronize");
+}
+
+} // namespace crd_amd::hesap::fft

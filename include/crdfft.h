@@ -85,7 +85,8 @@ typedef enum crdfft_algorithm
 
 /* Plan flags */
 #define CRDFFT_FLAG_NONE 0u
-#define CRDFFT_FLAG_NO_SELF_CHECK 1u /* skip the known-answer check at creation */
+#define CRDFFT_FLAG_NO_SELF_CHECK 1u       /* skip the known-answer check at creation */
+#define CRDFFT_FLAG_NORMALIZE_INVERSE 2u   /* C2C inverse scaled by 1/n (ifft_normalized, fft.hpp:1015-1029) */
 
 typedef struct crdfft_plan_s* crdfft_plan;
 
@@ -137,6 +138,20 @@ CRDFFT_API crdfft_status crdfft_execute_reference(crdfft_plan plan, const void* 
                                                   void* stream);
 /* Known-answer check of the plan's kernels (synchronous; allocates its own small buffers). */
 CRDFFT_API crdfft_status crdfft_self_check(crdfft_plan plan);
+
+/* Per-phase kernel times of one execute (the GPU counterpart of the reference's CRD_HESAP_FFT_PROFILE phase
+ * counters, fft_kernels.hpp:199-216, dumped by src/fft.cpp prof::dump): hipEvents recorded on `stream` around every
+ * launch, accumulated into *acc (zero it before the first call). Synchronises `stream` before returning.
+ * Phases: 0 = single kernel or four-step column pass, 1 = four-step row pass, 2 = real-transform pre/post pass,
+ * 3 = copies. points[] counts complex points read and written once by the phase's launches. */
+typedef struct crdfft_phase_times
+{
+    double ms[4];
+    uint64_t launches[4];
+    uint64_t points[4];
+} crdfft_phase_times;
+CRDFFT_API crdfft_status crdfft_execute_profiled(crdfft_plan plan, const void* in, void* out, int32_t direction,
+                                                 void* stream, crdfft_phase_times* acc);
 
 CRDFFT_API crdfft_status crdfft_plan_get_info(crdfft_plan plan, crdfft_plan_info* info);
 /* The planner's choice for a shape without creating a plan or touching a device. */

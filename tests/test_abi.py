@@ -1,0 +1,112 @@
+"""The C-ABI boundary on the CPU: the library loads, exports every declared entry point, the planner answers without
+a device, arguments are validated, and the C++20 interface compiles."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from crd_amd import _native as N
+
+    lib = N.lib()
+    names = N.header_symbols()
+    assert len(names) >= 30
+    missing = [s for s in names if not hasattr(lib, s)]
+    assert not missing, f"declared in include/*.h but not exported: {missing}"
+
+
+def test_python_binding_covers_every_symbol():
+    from crd_amd import _native as N
+
+    unbound = [s for s in N.header_symbols() if s not in N._SIGS]
+    assert not unbound, f"entry points without a ctypes signature: {unbound}"
+
+
+def test_version_and_status_strings():
+    from crd_amd import _native as N
+
+    assert N.lib().crdfft_version().startswith(b"crdfft")
+    for st in range(8):
+        assert N.status_string(st) != "unknown status"
+    assert N.status_string(99) == "unknown status"
+
+
+@pytest.mark.parametrize("lg", range(0, 25))
+def test_planner_covers_every_pow2(lg):
+    import crd_amd
+
+    n = 1 << lg
+    info = crd_amd.describe(n, 4)
+    assert info["complex_n"] == n
+    if n == 1:
+        assert info["algorithm"] == 0
+    elif n <= 16384:
+        assert info["algorithm"] == 1 and info["e"] >= 1
+    else:
+        assert info["algorithm"] == 2
+        assert info["n1"] * info["n2"] == n
+        assert info["workspace_bytes"] == info["chunk"] * n * 8
+        # the intermediate of one pass pair stays within the 128 MiB budget (Infinity Cache resident)
+        assert info["workspace_bytes"] <= (128 << 20) or info["chunk"] == 1
+
+
+def test_planner_real_transforms_use_half_length():
+    import crd_amd
+
+    for n in (4, 64, 1 << 16, 1 << 20):
+        for t in ("r2c", "c2r"):
+            assert crd_amd.describe(n, 2, t)["complex_n"] == n // 2
+
+
+def test_planner_rejects_bad_shapes():
+    import crd_amd
+    from crd_amd import _native as N
+
+    for n in (0, 3, 12, 1000, 1 << 25):
+        with pytest.raises(N.CrdfftError) as e:
+            crd_amd.describe(n, 1)
+        assert e.value.status == N.INVALID_SIZE
+    for n in (1, 2):
+        with pytest.raises(N.CrdfftError) as e:
+            crd_amd.describe(n, 1, "r2c")
+        assert e.value.status == N.INVALID_SIZE
+    info = N.PlanInfo()
+    assert N.lib().crdfft_plan_describe(64, 1, 7, 0, ctypes.byref(info)) == N.INVALID_ARGUMENT
+    assert N.lib().crdfft_plan_describe(64, 1, 0, 9, ctypes.byref(info)) == N.INVALID_ARGUMENT
+    assert N.lib().crdfft_plan_describe(64, 1, 0, 0, None) == N.INVALID_ARGUMENT
+
+
+def test_null_handles_are_rejected_without_a_device():
+    from crd_amd import _native as N
+
+    L = N.lib()
+    assert L.crdfft_plan_destroy(None) == N.INVALID_ARGUMENT
+    assert L.crdfft_execute(None, None, None, 0, None) == N.INVALID_ARGUMENT
+    assert L.crdfft_execute_batched(None, None, 1, 0, None) == N.INVALID_ARGUMENT
+    assert L.crdfft_self_check(None) == N.INVALID_ARGUMENT
+    assert L.crdfft_plan_create(None, 64, 1, 0, 0, 0, 0, 0) == N.INVALID_ARGUMENT
+
+
+def test_cpp_interface_compiles():
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = os.path.join(ROOT, "tests", "cpp", "wrapper_use.cpp")
+    r = subprocess.run([gxx, "-std=c++20", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_c_header_compiles_as_c():
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    code = '#include "crdfft.h"\n#include "crdfft_runtime.h"\nint main(void){crdfft_plan p = 0; (void)p; return 0;}\n'
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        "-x", "c", "-"], input=code, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,158 @@
+"""Pins the CPU oracle (oracle/): FP64 truth against numpy's pocketfft fixtures, closed forms and invariants; the FP32
+restatement against FP64 at the reference's tolerance classes; a planted mistake that must be caught."""
+import numpy as np
+import pytest
+
+C2C_SIZES = [1, 2, 3, 4, 5, 7, 8, 12, 16, 32, 60, 64, 97, 100, 128, 243, 256, 1000, 1024, 4096]
+R2C_SIZES = [4, 8, 16, 64, 256, 1024, 4096]
+
+
+def rel(y, ref):
+    den = np.linalg.norm(ref.ravel())
+    return float(np.linalg.norm((y - ref).ravel()) / (den if den > 0 else 1.0))
+
+
+def test_lcg_matches_fixture(po, golden):
+    for n in (1, 7, 64, 4096):
+        x = po.lcg_complex(n, 0x9E3779B97F4A7C15 ^ n, np.complex128)
+        assert np.array_equal(x, golden[f"c2c_in_{n}"])
+
+
+@pytest.mark.parametrize("n", C2C_SIZES)
+def test_fft64_matches_pocketfft(po, golden, n):
+    x = golden[f"c2c_in_{n}"]
+    assert rel(po.fft64(x), golden[f"c2c_fwd_{n}"]) < 1e-14
+    assert rel(po.fft64(x, inverse=True), golden[f"c2c_inv_{n}"]) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 12, 60, 97, 128, 243, 256])
+def test_dft64_matches_pocketfft(po, golden, n):
+    x = golden[f"c2c_in_{n}"]
+    assert rel(po.dft64(x), golden[f"c2c_fwd_{n}"]) < 1e-13
+    assert rel(po.dft64(x, inverse=True), golden[f"c2c_inv_{n}"]) < 1e-13
+
+
+@pytest.mark.parametrize("n", [1, 2, 6, 16, 97, 1000, 1024, 2187, 4096, 10007, 65536])
+def test_fft64_closed_forms(po, n):
+    # impulse at 0 -> all ones; constant -> n at bin 0; tone at f -> n at bin f; impulse at 1 -> W_n^k
+    imp = np.zeros(n, complex)
+    imp[0] = 1
+    assert np.max(np.abs(po.fft64(imp) - 1)) < 1e-12
+    const = np.ones(n, complex)
+    want = np.zeros(n, complex)
+    want[0] = n
+    assert np.max(np.abs(po.fft64(const) - want)) < 1e-12 * n
+    f = 3 % n
+    j = np.arange(n)
+    tone = np.exp(2j * np.pi * ((f * j) % n) / n)
+    want = np.zeros(n, complex)
+    want[f] = n
+    assert np.max(np.abs(po.fft64(tone) - want)) < 1e-11 * n
+    if n > 1:
+        imp1 = np.zeros(n, complex)
+        imp1[1] = 1
+        assert np.max(np.abs(po.fft64(imp1) - np.exp(-2j * np.pi * j / n))) < 1e-13 * np.log2(n) + 1e-15
+
+
+@pytest.mark.parametrize("n", [8, 60, 97, 1024, 3125, 16384])
+def test_fft64_invariants(po, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    X, Y = po.fft64(x), po.fft64(y)
+    # Parseval
+    assert abs(np.sum(np.abs(X) ** 2) - n * np.sum(np.abs(x) ** 2)) < 1e-10 * n * np.sum(np.abs(x) ** 2)
+    # linearity
+    assert rel(po.fft64(2.5 * x - 1j * y), 2.5 * X - 1j * Y) < 1e-13
+    # shift theorem: x[j - s] -> X[k] W^{s k}
+    s = 5 % n
+    k = np.arange(n)
+    assert rel(po.fft64(np.roll(x, s)), X * np.exp(-2j * np.pi * s * k / n)) < 1e-12
+    # inverse of forward is n * x
+    assert rel(po.fft64(X, inverse=True), n * x) < 1e-13
+
+
+@pytest.mark.parametrize("n", R2C_SIZES)
+def test_rfft64_matches_pocketfft(po, golden, n):
+    x = golden[f"r2c_in_{n}"]
+    X = po.rfft64(x)
+    assert rel(X, golden[f"r2c_out_{n}"]) < 1e-14
+    assert abs(X[0].imag) == 0 and abs(X[-1].imag) < 1e-12
+    assert np.max(np.abs(po.irfft64(X, n) - x)) < 1e-13
+
+
+TOL_CLASS = {False: 2e-4, True: 1e-3}  # max-rel, n < 2^19 and n >= 2^19 (test_fft_kernels.cpp:151-163)
+
+
+@pytest.mark.parametrize("lg", list(range(0, 18)) + [19, 20])
+def test_plan32_against_fp64(po, lg):
+    n = 1 << lg
+    x = po.lcg_complex(n, 0x12345 ^ n)
+    p = po.Plan32(n)
+    for inv in (False, True):
+        y = p.execute(x, inv)
+        ref = po.fft64(x.astype(np.complex128), inverse=inv)
+        assert po.max_rel(y, ref) < TOL_CLASS[n >= (1 << 19)]
+        assert po.rel_l2(y, ref) <= 2e-6 * max(lg, 1)
+        yr = p.execute_reference(x, inv)
+        assert po.rel_l2(yr, ref) <= 2e-6 * max(lg, 1)
+
+
+def test_plan32_known_answers(po, golden):
+    """The reference self-check's integer inputs: FP32 outputs within 1.0e-7 of max|X| (fft_self_check.cpp:4-6)."""
+    for n in (2, 4, 8, 16, 32, 64, 128):
+        x = golden[f"kat_in_{n}"]
+        p = po.Plan32(n)
+        for inv, key in ((False, "fwd"), (True, "inv")):
+            ref = golden[f"kat_{key}_{n}"]
+            y = p.execute(x.astype(np.complex64), inv)
+            assert np.max(np.abs(y - ref)) <= 1.0e-7 * np.max(np.abs(ref)) * max(1, np.log2(n) / 2)
+
+
+def test_planted_twiddle_sign_is_caught(po):
+    """A conjugated twiddle table (the classic sign slip) must fail the parity bar by orders of magnitude."""
+    for n in (64, 4096, 1 << 17):
+        x = po.lcg_complex(n, 99)
+        ref = po.fft64(x.astype(np.complex128))
+        good = po.Plan32(n).execute(x)
+        bad = po.Plan32(n, planted=True).execute(x)
+        assert po.rel_l2(good, ref) <= 2e-6 * np.log2(n)
+        assert po.rel_l2(bad, ref) > 0.1
+
+
+def test_plan32_element_major_batch(po):
+    for n, batch in ((8, 7), (64, 19), (1024, 5)):
+        x = po.lcg_complex(n * batch, n + batch)
+        y = po.Plan32(n).execute_batched(x, batch)
+        ref = np.fft.fft(x.reshape(n, batch).astype(np.complex128), axis=0)
+        assert po.rel_l2(y.reshape(n, batch), ref) <= 2e-6 * np.log2(n)
+
+
+def test_batch_split_is_bit_identical_across_jobs(po):
+    """run_batch: every job count gives the serial bits (batch_arms.hpp:11-14)."""
+    for n in (64, 4096, 1 << 17):
+        x = po.lcg_complex(n * 9, 5).reshape(9, n)
+        ys = [po.c2c32(x, threads=t) for t in (1, 2, 3, 8)]
+        for y in ys[1:]:
+            assert np.array_equal(y.view(np.uint32), ys[0].view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 12, 97, 100, 1000, 4096 + 1, 7 * 11 * 13])
+def test_bluestein32_against_fp64(po, n):
+    x = po.lcg_complex(n, 17 * n)
+    for inv in (False, True):
+        y = po.bluestein32(x, inv)
+        ref = po.fft64(x.astype(np.complex128), inverse=inv)
+        if inv:
+            ref = ref / n  # the reference's Bluestein inverse is normalized (bluestein.hpp:86)
+        assert po.rel_l2(y, ref) <= 2e-6 * np.log2(n) * 2
+
+
+@pytest.mark.parametrize("n", [4, 8, 64, 1024, 1 << 16, 1 << 20])
+def test_real32_against_fp64(po, n):
+    x = po.normal_real(3 * n, 11).reshape(3, n)
+    X = po.rfft32(x)
+    ref = po.rfft64(x.astype(np.float64))
+    assert po.rel_l2(X, ref) <= 2e-6 * np.log2(n)
+    back = po.irfft32(X, n)
+    assert np.max(np.abs(back - x)) <= 2e-6 * np.log2(n) * np.max(np.abs(x))
