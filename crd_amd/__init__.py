@@ -5,11 +5,11 @@ include/crdfft.h. This package is the Python side of that ABI: ctypes bindings a
 FftPlan / RealFftPlan / fft / ifft_normalized interface.
 """
 from ._native import (C2C, C2R, FORWARD, INVERSE, R2C, CrdfftError, LIB_PATH, lib)  # noqa: F401
-from .fft import (DeviceBuffer, Event, FftDirection, FftPlan, RealFftPlan, Stream, describe, device_count,  # noqa
+from .fft import (BluesteinPlan, DeviceBuffer, NdFftPlan, Event, FftDirection, FftPlan, RealFftPlan, Stream, describe, device_count,  # noqa
                   device_props, fft, ifft_normalized, synchronize)
 
 __all__ = [
-    "FftPlan", "RealFftPlan", "DeviceBuffer", "Stream", "Event", "FftDirection", "describe", "device_count",
+    "FftPlan", "RealFftPlan", "BluesteinPlan", "NdFftPlan", "DeviceBuffer", "Stream", "Event", "FftDirection", "describe", "device_count",
     "device_props", "fft", "ifft_normalized", "synchronize", "CrdfftError", "lib", "LIB_PATH", "FORWARD",
     "INVERSE", "C2C", "R2C", "C2R",
 ]

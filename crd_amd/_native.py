@@ -16,7 +16,7 @@ OK, INVALID_SIZE, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, SELF_CHECK_FAILED,
 FORWARD, INVERSE = 0, 1
 C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
-ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE = 0, 1, 2, 3
+ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN = 0, 1, 2, 3, 4
 FLAG_NONE, FLAG_NO_SELF_CHECK = 0, 1
 
 
@@ -47,6 +47,7 @@ class PlanInfo(ctypes.Structure):
         ("chunk", ctypes.c_uint64),
         ("workspace_bytes", ctypes.c_uint64),
         ("table_bytes", ctypes.c_uint64),
+        ("hints", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -62,6 +63,7 @@ class Tuning(ctypes.Structure):
         ("e2", ctypes.c_int32),
         ("tile2", ctypes.c_int32),
         ("chunk", ctypes.c_uint64),
+        ("hints", ctypes.c_uint32),
     ]
 
 
@@ -98,12 +100,16 @@ _SIGS = {
     "crdfft_plan_create_tuned": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _i32, _i32, _u32,
                                         ctypes.POINTER(Tuning)]),
     "crdfft_plan_destroy": (_i32, [_vp]),
+    "crdfft_bluestein_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _u32]),
     "crdfft_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_execute_batched": (_i32, [_vp, _vp, _u64, _i32, _vp]),
     "crdfft_execute_reference": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_self_check": (_i32, [_vp]),
     "crdfft_execute_profiled": (_i32, [_vp, _vp, _vp, _i32, _vp, ctypes.POINTER(PhaseTimes)]),
     "crdfft_plan_get_info": (_i32, [_vp, ctypes.POINTER(PlanInfo)]),
+    "crdfft_nd_plan_create": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(_u64), _u32, _u64, _u32]),
+    "crdfft_nd_execute": (_i32, [_vp, _vp, _i32, _vp]),
+    "crdfft_nd_plan_destroy": (_i32, [_vp]),
     "crdfft_plan_describe": (_i32, [_u64, _u64, _i32, _i32, ctypes.POINTER(PlanInfo)]),
     "crdfft_status_string": (ctypes.c_char_p, [_i32]),
     "crdfft_last_hip_error": (_i32, []),
@@ -116,6 +122,7 @@ _SIGS = {
     "crdfft_memcpy_d2h": (_i32, [_vp, _vp, _u64]),
     "crdfft_memcpy_d2d": (_i32, [_vp, _vp, _u64, _vp]),
     "crdfft_fill_bytes": (_i32, [_vp, _i32, _u64, _vp]),
+    "crdfft_stream_copy": (_i32, [_vp, _vp, _u64, _vp]),
     "crdfft_stream_create": (_i32, [ctypes.POINTER(_vp)]),
     "crdfft_stream_destroy": (_i32, [_vp]),
     "crdfft_stream_synchronize": (_i32, [_vp]),

@@ -44,6 +44,79 @@ __device__ __forceinline__ uint64_t xcd_block()
 
 constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT> __device__ __forceinline__ cf ldg(const cf* p)
+{
+    if constexpr (NT)
+    {
+        const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+        return cf{v.x, v.y};
+    }
+    else
+    {
+        return *p;
+    }
+}
+
+template <bool NT> __device__ __forceinline__ void stg(cf* p, cf v)
+{
+    if constexpr (NT)
+    {
+        f2v w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+    }
+    else
+    {
+        *p = v;
+    }
+}
+
+// v[m] = src[m * stride] for m < E, with or without the non-temporal hint (a uniform branch).
+template <int E>
+__device__ __forceinline__ void load_strided(cf (&v)[E], const cf* src, uint64_t stride, bool nt)
+{
+    if (nt)
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = ldg<true>(src + m * stride);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = ldg<false>(src + m * stride);
+        }
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void store_strided(const cf (&v)[E], cf* dst, uint64_t stride, float scale, bool nt)
+{
+    if (nt)
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            stg<true>(dst + m * stride, cf{v[m].x * scale, v[m].y * scale});
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            stg<false>(dst + m * stride, cf{v[m].x * scale, v[m].y * scale});
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // k_rows
 // ---------------------------------------------------------------------------------------------------------------
@@ -72,12 +145,7 @@ __global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
     cf v[E];
     if (active)
     {
-        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
-#pragma unroll
-        for (int m = 0; m < E; ++m)
-        {
-            v[m] = src[i + C::T * m];
-        }
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + i, C::T, (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -91,12 +159,8 @@ __global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
     fft_core<N, E, INV, 1>(v, i, view, reinterpret_cast<const cf*>(a.tw));
     if (active)
     {
-        cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
-#pragma unroll
-        for (int m = 0; m < E; ++m)
-        {
-            dst[i + C::T * m] = cf{v[m].x * a.scale, v[m].y * a.scale};
-        }
+        store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + i, C::T, a.scale,
+                         (a.hints & kHintNtStore) != 0);
     }
 }
 
@@ -128,12 +192,8 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     cf v[E];
     if (active)
     {
-        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist + col;
-#pragma unroll
-        for (int m = 0; m < E; ++m)
-        {
-            v[m] = src[static_cast<uint64_t>(i + C::T * m) * a.ncols];
-        }
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + col + static_cast<uint64_t>(i) * a.ncols,
+                        C::T * a.ncols, (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -151,23 +211,24 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     }
     if (a.thi != nullptr)
     {
+        // W_N^{col (i + T m)} = W_N^{col i} * W_N^{col T m}: one factored lookup per thread, then one coalesced
+        // table load per value (lanes of adjacent columns read adjacent entries)
         const cf* thi = reinterpret_cast<const cf*>(a.thi);
         const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+        const cf* tstep = reinterpret_cast<const cf*>(a.tstep) + col;
         const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
+        const uint64_t e0 = col * static_cast<uint64_t>(i);
+        const cf w0 = cmul(thi[e0 >> a.hshift], tlo[e0 & lmask]);
+        v[0] = ctw<INV>(v[0], w0);
 #pragma unroll
-        for (int m = 0; m < E; ++m)
+        for (int m = 1; m < E; ++m)
         {
-            const uint64_t e = col * static_cast<uint64_t>(i + C::T * m);
-            const cf w = cmul(thi[e >> a.hshift], tlo[e & lmask]);
+            const cf w = cmul(w0, tstep[static_cast<uint64_t>(m) * a.ncols]);
             v[m] = ctw<INV>(v[m], w);
         }
     }
-    cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist + col;
-#pragma unroll
-    for (int m = 0; m < E; ++m)
-    {
-        dst[static_cast<uint64_t>(i + C::T * m) * a.ncols] = v[m];
-    }
+    store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + col + static_cast<uint64_t>(i) * a.ncols,
+                     C::T * a.ncols, 1.0f, (a.hints & kHintNtStore) != 0);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -201,12 +262,8 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     cf v[E];
     if (active)
     {
-        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2;
-#pragma unroll
-        for (int m = 0; m < E; ++m)
-        {
-            v[m] = src[i + C::T * m];
-        }
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2 + i, C::T,
+                        (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -239,13 +296,15 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     }
     const int r2 = tid % RR;
     const int j2 = tid / RR;
-    cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2;
+    cf w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m)
     {
         const int k2 = j2 + C::T * m;
-        dst[static_cast<uint64_t>(k2) * a.n1] = cf{tre[k2 * C::TP + r2] * a.scale, tim[k2 * C::TP + r2] * a.scale};
+        w[m] = cf{tre[k2 * C::TP + r2], tim[k2 * C::TP + r2]};
     }
+    store_strided<E>(w, reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2 + static_cast<uint64_t>(j2) * a.n1,
+                     C::T * a.n1, a.scale, (a.hints & kHintNtStore) != 0);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -271,6 +330,18 @@ __global__ void k_roots_lin(cf* out, uint64_t count, uint64_t mult, uint64_t d)
     if (j < count)
     {
         out[j] = root_fp64((j * mult) % d, d);
+    }
+}
+
+// Step table of the column pass: out[m * ncols + n2] = W_n^{n2 * step * m}.
+__global__ void k_roots_step(cf* out, uint64_t ncols, uint64_t e, uint64_t step, uint64_t n)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx < ncols * e)
+    {
+        const uint64_t m = idx / ncols;
+        const uint64_t n2 = idx - m * ncols;
+        out[idx] = root_fp64((n2 * step % n) * m, n);
     }
 }
 
@@ -323,6 +394,15 @@ __global__ void k_ref_stage(cf* data, const cf* w, uint64_t n, uint64_t len, uin
     const cf av = base[k];
     base[k] = cadd(av, b);
     base[k + half] = csub(av, b);
+}
+
+__global__ void k_stream_copy(const float4* __restrict__ in, float4* __restrict__ out, uint64_t count)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    {
+        out[i] = in[i];
+    }
 }
 
 __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64_t idist, uint64_t odist)
@@ -551,6 +631,18 @@ hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, h
     return hipGetLastError();
 }
 
+hipError_t build_step_table(cfloat* out, uint64_t ncols, int e, uint64_t step, uint64_t n, hipStream_t s)
+{
+    const uint64_t count = ncols * static_cast<uint64_t>(e);
+    if (count == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_roots_step, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<cf*>(out), ncols, static_cast<uint64_t>(e), step, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint64_t n, uint64_t count,
                             uint64_t idist, uint64_t odist, bool inverse, hipStream_t s)
 {
@@ -577,6 +669,23 @@ hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((bfly + 255) / 256)), dim3(256), 0, s,
                            reinterpret_cast<cf*>(out), reinterpret_cast<const cf*>(w), n, len, count, odist);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s)
+{
+    if (bytes % 16 != 0)
+    {
+        return hipErrorInvalidValue;
+    }
+    const uint64_t count = bytes / 16;
+    const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>((count + 255) / 256, 256 * 32));
+    if (count == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(src),
+                       static_cast<float4*>(dst), count);
     return hipGetLastError();
 }
 

@@ -25,7 +25,13 @@ struct RowArgs
     uint64_t idist;
     uint64_t odist;
     float scale; // applied to every output (1 = unscaled, exact)
+    uint32_t hints; // kHintNtLoad / kHintNtStore
 };
+
+// Cache hints: non-temporal loads of the input / stores of the output (streamed data that should not displace a
+// cache-resident intermediate).
+constexpr uint32_t kHintNtLoad = 1u;
+constexpr uint32_t kHintNtStore = 2u;
 
 // Column transforms of length n1 over `ncols` adjacent columns of a row-major n1 x ncols matrix, for `count`
 // matrices at t*dist: four-step pass 1 (with the factored twiddle W_N^{n2 k1} when thi != nullptr) and the
@@ -35,13 +41,15 @@ struct ColArgs
     const cfloat* in;
     cfloat* out;
     const cfloat* tw;  // stage twiddles of n1
-    const cfloat* thi; // W_N^{j << hshift}
-    const cfloat* tlo; // W_N^{j}, j < 2^hshift
+    const cfloat* thi;   // W_N^{j << hshift}
+    const cfloat* tlo;   // W_N^{j}, j < 2^hshift
+    const cfloat* tstep; // W_N^{n2 * (n1/E) * m} at [m * ncols + n2], m < E (the per-slot step of the twiddle)
     uint32_t hshift;
     uint64_t ncols;
     uint64_t count;
     uint64_t idist;
     uint64_t odist;
+    uint32_t hints;
 };
 
 // Row transforms of length n2 over the n1 rows of a row-major n1 x n2 matrix, stored transposed:
@@ -56,6 +64,7 @@ struct RowTrArgs
     uint64_t idist;
     uint64_t odist;
     float scale; // applied to every output (1 = unscaled, exact)
+    uint32_t hints;
 };
 
 struct LaunchShape
@@ -80,12 +89,32 @@ hipError_t build_stage_twiddles(cfloat* table, uint64_t n, int e, hipStream_t s)
 // out[j] = W_d^{j * mult} for j < count, computed in FP64 on the device and narrowed.
 hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, hipStream_t s);
 
+// The column pass's step table: out[m * ncols + n2] = W_n^{n2 * step * m}, m < e (step = n1 / e).
+hipError_t build_step_table(cfloat* out, uint64_t ncols, int e, uint64_t step, uint64_t n, hipStream_t s);
+
+// Bluestein passes (bluestein_kernels.hip): chirp pre-multiply + zero pad, pointwise product with FFT_m(b), chirp
+// post-multiply (with the inverse's conjugation and 1/n), and the FP64-built chirp / b tables.
+hipError_t launch_blue_pre(const cfloat* x, cfloat* w, const cfloat* chirp, uint64_t n, uint64_t m, uint64_t count,
+                           uint64_t idist, bool inverse, hipStream_t s);
+hipError_t launch_blue_mul(cfloat* w, const cfloat* bhat, uint64_t m, uint64_t count, hipStream_t s);
+hipError_t launch_blue_post(const cfloat* w, cfloat* y, const cfloat* chirp, uint64_t n, uint64_t m, uint64_t count,
+                            uint64_t odist, bool inverse, hipStream_t s);
+hipError_t build_blue_tables(cfloat* chirp, cfloat* b, uint64_t n, uint64_t m, hipStream_t s);
+
+// out[t][c][r] = in[t][r][c] (nd_kernels.hip), and data *= scale.
+hipError_t launch_transpose(const cfloat* in, cfloat* out, uint64_t rows, uint64_t cols, uint64_t count,
+                            uint64_t idist, uint64_t odist, hipStream_t s);
+hipError_t launch_scale(cfloat* data, uint64_t count, float scale, hipStream_t s);
+
 // The radix-2 reference transform (bit reversal + log2 n butterfly stages over global memory) of `count`
 // contiguous transforms; `w` holds W_n^k for k < n/2.
 hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint64_t n, uint64_t count,
                             uint64_t idist, uint64_t odist, bool inverse, hipStream_t s);
 
 // out[t*odist + i] = in[t*idist + i] for i < n (n = 1 transforms, and host-side copies of a batch).
+// Streaming copy of `bytes` (a multiple of 16) with 16-byte vector accesses: the device-copy ceiling probe.
+hipError_t launch_stream_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+
 hipError_t launch_copy(const cfloat* in, cfloat* out, uint64_t n, uint64_t count, uint64_t idist, uint64_t odist,
                        hipStream_t s);
 

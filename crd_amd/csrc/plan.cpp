@@ -90,12 +90,24 @@ struct crdfft_plan_s
     DeviceBuffer tw_single; // stage table of the single kernel (complex_n)
     DeviceBuffer tw1, tw2;  // stage tables of the column (n1) and row (n2) kernels
     DeviceBuffer thi, tlo;  // factored four-step twiddle
+    DeviceBuffer tstep;     // column pass step table (e1 * n2 entries)
     uint32_t hshift = 0;
     DeviceBuffer wreal;     // W_n^k, k <= n/2, for R2C/C2R
     DeviceBuffer wref;      // W_cn^k, k < cn/2, for the radix-2 reference (built on first use)
     DeviceBuffer workspace; // four-step intermediate, chunk * complex_n
     DeviceBuffer tw_em;     // stage table of the element-major kernel (built on first use)
+    // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
+    crdfft_plan inner = nullptr;
+    DeviceBuffer chirp, bhat, bwork;
+    uint64_t blue_chunk = 0;
     std::mutex ref_mutex;
+    ~crdfft_plan_s()
+    {
+        if (inner != nullptr)
+        {
+            crdfft_plan_destroy(inner);
+        }
+    }
 };
 
 namespace
@@ -196,6 +208,8 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
         CRDFFT_TRY(p->tlo.alloc(lo * sizeof(cfloat)));
         CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), hi, lo, cn, s));
         CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
+        CRDFFT_TRY(p->tstep.alloc(static_cast<uint64_t>(sh.e1) * sh.n2 * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_step_table(p->tstep.as<cfloat>(), sh.n2, sh.e1, sh.n1 / sh.e1, cn, s));
         CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
     }
     if (sh.type != CRDFFT_C2C)
@@ -225,7 +239,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             return CRDFFT_OK;
         case CRDFFT_ALGO_SINGLE:
         {
-            crdfft::RowArgs a{in, out, p->tw_single.as<cfloat>(), count, idist, odist, scale};
+            crdfft::RowArgs a{in, out, p->tw_single.as<cfloat>(), count, idist, odist, scale, sh.hints};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_rows(cn, sh.e, inverse, a, s));
             CRDFFT_TRY(prof.after(0, cn * count));
@@ -241,16 +255,18 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                    p->tw1.as<cfloat>(),
                                    p->thi.as<cfloat>(),
                                    p->tlo.as<cfloat>(),
+                                   p->tstep.as<cfloat>(),
                                    p->hshift,
                                    sh.n2,
                                    cnt,
                                    idist,
-                                   cn};
+                                   cn,
+                                   sh.hints & crdfft::kHintNtLoad};
                 CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
                 CRDFFT_TRY(prof.after(0, cn * cnt));
                 crdfft::RowTrArgs ra{p->workspace.as<cfloat>(), out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
-                                     cn, odist, scale};
+                                     cn, odist, scale, sh.hints & crdfft::kHintNtStore};
                 CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
                 CRDFFT_TRY(prof.after(1, cn * cnt));
@@ -279,6 +295,39 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
     if (sh.precision != CRDFFT_FP32)
     {
         return CRDFFT_UNSUPPORTED;
+    }
+    if (sh.algorithm == CRDFFT_ALGO_BLUESTEIN)
+    {
+        const uint64_t n = sh.n, m = sh.n1;
+        const cfloat* x = static_cast<const cfloat*>(in);
+        cfloat* y = static_cast<cfloat*>(out);
+        cfloat* w = p->bwork.as<cfloat>();
+        for (uint64_t c0 = 0; c0 < sh.batch; c0 += p->blue_chunk)
+        {
+            const uint64_t cnt = std::min(p->blue_chunk, sh.batch - c0);
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_blue_pre(x + c0 * p->idist, w, p->chirp.as<cfloat>(), n, m, cnt, p->idist,
+                                               inverse, s));
+            CRDFFT_TRY(prof.after(2, n * cnt));
+            crdfft_status st = run_complex(p->inner, w, w, cnt, m, m, false, s, prof);
+            if (st != CRDFFT_OK)
+            {
+                return st;
+            }
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_blue_mul(w, p->bhat.as<cfloat>(), m, cnt, s));
+            CRDFFT_TRY(prof.after(2, m * cnt));
+            st = run_complex(p->inner, w, w, cnt, m, m, true, s, prof);
+            if (st != CRDFFT_OK)
+            {
+                return st;
+            }
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_blue_post(w, y + c0 * p->odist, p->chirp.as<cfloat>(), n, m, cnt, p->odist,
+                                                inverse, s));
+            CRDFFT_TRY(prof.after(2, n * cnt));
+        }
+        return CRDFFT_OK;
     }
     switch (sh.type)
     {
@@ -412,6 +461,88 @@ crdfft_status crdfft_plan_create_tuned(crdfft_plan* plan, uint64_t n, uint64_t b
     return create_plan(plan, n, batch, idist, odist, type, precision, flags, tuning);
 }
 
+crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
+                                           uint64_t odist, uint32_t flags)
+{
+    if (plan == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *plan = nullptr;
+    if (n == 0)
+    {
+        return CRDFFT_INVALID_SIZE;
+    }
+    uint64_t m = 1;
+    while (m < 2 * n - 1)
+    {
+        m <<= 1;
+    }
+    if (m > crdfft::kFourStepMax)
+    {
+        return CRDFFT_INVALID_SIZE;
+    }
+    idist = idist == 0 ? n : idist;
+    odist = odist == 0 ? n : odist;
+    if (batch > 1 && (idist < n || odist < n))
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    // workspace chunk: at most 256 MiB of m-point rows, at least one
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint64_t>(batch, 1),
+                                                                    crdfft::kChunkBytes / (m * 8)));
+    crdfft_plan inner = nullptr;
+    crdfft_status st =
+        crdfft_plan_create(&inner, m, chunk, m, m, CRDFFT_C2C, CRDFFT_FP32, flags | CRDFFT_FLAG_NO_SELF_CHECK);
+    if (st != CRDFFT_OK)
+    {
+        return st;
+    }
+    auto* p = new (std::nothrow) crdfft_plan_s;
+    if (p == nullptr)
+    {
+        crdfft_plan_destroy(inner);
+        return CRDFFT_OUT_OF_MEMORY;
+    }
+    p->inner = inner;
+    p->shape.n = n;
+    p->shape.batch = batch;
+    p->shape.type = CRDFFT_C2C;
+    p->shape.precision = CRDFFT_FP32;
+    p->shape.algorithm = CRDFFT_ALGO_BLUESTEIN;
+    p->shape.complex_n = n;
+    p->shape.n1 = m;
+    p->shape.chunk = chunk;
+    p->shape.workspace_bytes = chunk * m * 8;
+    p->idist = idist;
+    p->odist = odist;
+    p->flags = flags;
+    p->blue_chunk = chunk;
+    (void)hipGetDevice(&p->device);
+    auto fail = [&](crdfft_status e) {
+        delete p;
+        return e;
+    };
+    if (hipError_t e = p->chirp.alloc(n * 8); e != hipSuccess)
+        return fail(hip_status(e));
+    if (hipError_t e = p->bhat.alloc(m * 8); e != hipSuccess)
+        return fail(hip_status(e));
+    if (hipError_t e = p->bwork.alloc(chunk * m * 8); e != hipSuccess)
+        return fail(hip_status(e));
+    if (hipError_t e = crdfft::build_blue_tables(p->chirp.as<cfloat>(), p->bhat.as<cfloat>(), n, m, nullptr);
+        e != hipSuccess)
+        return fail(hip_status(e));
+    // FFT_m(b) through the inner plan's kernels (the reference transforms b with its own plan, bluestein.hpp:67)
+    Profiler none;
+    st = run_complex(inner, p->bhat.as<cfloat>(), p->bhat.as<cfloat>(), 1, m, m, false, nullptr, none);
+    if (st != CRDFFT_OK)
+        return fail(st);
+    if (hipError_t e = hipStreamSynchronize(nullptr); e != hipSuccess)
+        return fail(hip_status(e));
+    *plan = p;
+    return CRDFFT_OK;
+}
+
 crdfft_status crdfft_plan_destroy(crdfft_plan plan)
 {
     if (plan == nullptr)
@@ -490,7 +621,8 @@ crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, 
         }
         tw = p->tw_em.as<cfloat>();
     }
-    crdfft::ColArgs a{static_cast<const cfloat*>(data), static_cast<cfloat*>(data), tw, nullptr, nullptr, 0, batch, 1, 0, 0};
+    crdfft::ColArgs a{static_cast<const cfloat*>(data), static_cast<cfloat*>(data), tw, nullptr, nullptr, nullptr, 0, batch,
+                      1, 0, 0, 0};
     CRDFFT_TRY(crdfft::launch_cols(sh.complex_n, e, c, inverse, a, s));
     return CRDFFT_OK;
 }
@@ -531,7 +663,7 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
     }
     crdfft::shape_to_info(p->shape, info);
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
-                        p->wreal.bytes;
+                        p->tstep.bytes + p->wreal.bytes;
     return CRDFFT_OK;
 }
 
@@ -648,6 +780,16 @@ crdfft_status crdfft_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
 crdfft_status crdfft_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream)
 {
     CRDFFT_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_stream_copy(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    if ((dst == nullptr || src == nullptr) && bytes != 0)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    CRDFFT_TRY(crdfft::launch_stream_copy(dst, src, bytes, static_cast<hipStream_t>(stream)));
     return CRDFFT_OK;
 }
 

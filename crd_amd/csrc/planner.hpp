@@ -28,15 +28,19 @@ struct PlanShape
     int32_t e2 = 0, tile2 = 0;
     uint64_t chunk = 0;
     uint64_t workspace_bytes = 0;
+    uint32_t hints = 0; // kHintNtLoad | kHintNtStore
 };
 
 // The largest complex length one LDS-resident kernel runs.
 constexpr uint64_t kSingleMax = 16384;
 // The largest complex length of the plain path (two LDS-resident passes of at most 4096 each).
 constexpr uint64_t kFourStepMax = uint64_t{1} << 24;
-// Bytes of four-step intermediate kept in flight between the column and the row pass: small enough that the
-// intermediate stays in the 256 MB Infinity Cache, large enough that each pass fills the GPU.
-constexpr uint64_t kChunkBytes = uint64_t{128} << 20;
+// Bytes of four-step intermediate per column/row pass pair: each pass then launches at least 32K workgroups, so
+// launch gaps and the partial last wave cost little (measured on MI355X: 64 MiB chunks ran 15% slower than 128 MiB
+// and those slower than 256 MiB at n = 2^16; the intermediate does not measurably stay in the Infinity Cache).
+constexpr uint64_t kChunkBytes = uint64_t{256} << 20;
+// Cache hints of the four-step: bit 0 streams the column pass's input past the caches, bit 1 the row pass's output.
+constexpr uint32_t kDefaultFourStepHints = 3;
 
 bool is_pow2(uint64_t n);
 uint32_t log2_exact(uint64_t n);

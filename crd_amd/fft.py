@@ -225,6 +225,43 @@ class FftPlan:
             pass
 
 
+class BluesteinPlan(FftPlan):
+    """Any-length C2C (bluestein.hpp:29-136): chirp-z over a power-of-two plan of m >= 2n-1; inverse scaled 1/n."""
+
+    def __init__(self, n, batch=1, idist=0, odist=0):
+        self.n = int(n)
+        self.batch = int(batch)
+        self.type = "c2c"
+        self.handle = ctypes.c_void_p()
+        check(N.lib().crdfft_bluestein_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist, 0),
+              f"crdfft_bluestein_plan_create(n={n}, batch={batch})")
+
+    def conv_size(self):
+        return self.info()["n1"]
+
+
+class NdFftPlan:
+    """Row-major N-D C2C of `batch` volumes in place (nd_fft.hpp:28-196): inverse scaled by 1/prod(dims)."""
+
+    def __init__(self, dims, batch=1):
+        self.dims = [int(d) for d in dims]
+        arr = (ctypes.c_uint64 * len(self.dims))(*self.dims)
+        self.handle = ctypes.c_void_p()
+        check(N.lib().crdfft_nd_plan_create(ctypes.byref(self.handle), arr, len(self.dims), int(batch), 0),
+              f"crdfft_nd_plan_create({self.dims})")
+
+    def execute(self, data, direction=N.FORWARD, stream=None):
+        check(N.lib().crdfft_nd_execute(self.handle, _ptr(data), int(direction), _stream(stream)),
+              "crdfft_nd_execute")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                N.lib().crdfft_nd_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
 class RealFftPlan:
     """real[n] <-> complex[n/2+1] for `batch` transforms (real_fft.hpp:18-101)."""
 

@@ -80,7 +80,8 @@ typedef enum crdfft_algorithm
     CRDFFT_ALGO_COPY = 0,       /* n == 1 */
     CRDFFT_ALGO_SINGLE = 1,     /* one LDS-staged kernel per transform (n <= 16384) */
     CRDFFT_ALGO_FOUR_STEP = 2,  /* n = n1*n2: column pass with twiddle, row pass with transposed store */
-    CRDFFT_ALGO_REFERENCE = 3   /* the radix-2 reference only */
+    CRDFFT_ALGO_REFERENCE = 3,  /* the radix-2 reference only */
+    CRDFFT_ALGO_BLUESTEIN = 4   /* chirp-z over a power-of-two plan of m >= 2n-1 (any n) */
 } crdfft_algorithm;
 
 /* Plan flags */
@@ -106,6 +107,7 @@ typedef struct crdfft_plan_info
     uint64_t chunk;       /* transforms per column+row pass pair (four-step) */
     uint64_t workspace_bytes;
     uint64_t table_bytes;
+    uint32_t hints;       /* cache hints: bit 0 non-temporal input loads, bit 1 non-temporal output stores */
 } crdfft_plan_info;
 
 /* Tuning overrides for crdfft_plan_create_tuned; a zero field keeps the planner's choice. */
@@ -114,6 +116,7 @@ typedef struct crdfft_tuning
     uint64_t n1;          /* four-step split */
     int32_t e, e1, tile1, e2, tile2;
     uint64_t chunk;       /* transforms per pass pair */
+    uint32_t hints;       /* cache hints (crdfft_plan_info.hints); 0 keeps the planner's */
 } crdfft_tuning;
 
 /* A plan of `batch` transforms of length n. idist/odist: distance between consecutive transforms in elements of
@@ -125,6 +128,11 @@ CRDFFT_API crdfft_status crdfft_plan_create_tuned(crdfft_plan* plan, uint64_t n,
                                                   uint64_t odist, int32_t type, int32_t precision, uint32_t flags,
                                                   const crdfft_tuning* tuning);
 CRDFFT_API crdfft_status crdfft_plan_destroy(crdfft_plan plan);
+/* BluesteinPlan<T>(IAllocator*, usize n) (bluestein.hpp:29-70): a C2C plan of any n >= 1 run as a chirp-z
+ * transform over a power-of-two plan of m >= 2n-1 (crdfft_plan_info.n1 = m). Forward unnormalized; unlike FftPlan,
+ * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. */
+CRDFFT_API crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
+                                                      uint64_t odist, uint32_t flags);
 
 /* The plan's transforms on device memory, asynchronously on `stream`. in == out runs in place (C2C only). */
 CRDFFT_API crdfft_status crdfft_execute(crdfft_plan plan, const void* in, void* out, int32_t direction,
@@ -157,6 +165,137 @@ CRDFFT_API crdfft_status crdfft_plan_get_info(crdfft_plan plan, crdfft_plan_info
 /* The planner's choice for a shape without creating a plan or touching a device. */
 CRDFFT_API crdfft_status crdfft_plan_describe(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
                                               crdfft_plan_info* info);
+
+/* ---- N-dimensional transforms: NdFftPlan<T>(IAllocator*, ConstSpan<usize> dims) (nd_fft.hpp:28-196) ----
+ * Row-major volumes of `ndim` dims (each >= 1), `batch` volumes back to back, in place. Forward is the unnormalized
+ * N-D DFT; the inverse is scaled by 1/prod(dims) (nd_fft.hpp:104). Power-of-two axes up to 4096 run as strided
+ * column transforms in place; other axes are transposed to contiguous lines, transformed by a 1D plan (power of two,
+ * else Bluestein, as the reference picks, nd_fft.hpp:66-78) and transposed back. */
+typedef struct crdfft_nd_plan_s* crdfft_nd_plan;
+#define CRDFFT_ND_TRANSPOSE_ALL 1u /* flag: run every axis but the last through transposed lines */
+CRDFFT_API crdfft_status crdfft_nd_plan_create(crdfft_nd_plan* plan, const uint64_t* dims, uint32_t ndim,
+                                               uint64_t batch, uint32_t flags);
+CRDFFT_API crdfft_status crdfft_nd_execute(crdfft_nd_plan plan, void* data, int32_t direction, void* stream);
+CRDFFT_API crdfft_status crdfft_nd_plan_destroy(crdfft_nd_plan plan);
+
+/* ---- mixed-size batches: one persistent launch over transforms of many 2/3/5/7-smooth lengths ----
+ * No single reference call does this: the reference runs such a batch as one plan per length, through Bluestein for
+ * every length that is not a power of two (bluestein.hpp), one job per contiguous range (detail/batch_arms.hpp).
+ * Here the transforms are grouped by length into tiles, the tiles balanced over one workgroup per CU, and each length
+ * runs its own radix plan (radices 2, 3, 4, 5, 7, 8, 16) chosen by the autotuner. C2C, FP32, unnormalized. */
+typedef struct crdfft_many_plan_s* crdfft_many_plan;
+
+#define CRDFFT_MAX_STAGES 16
+/* A radix plan of one length: the stage radices in order (product n) and transforms per tile (0 = as many as fit
+ * 16384 points). */
+typedef struct crdfft_radix_plan
+{
+    uint64_t n;
+    uint32_t nstages;
+    uint32_t radix[CRDFFT_MAX_STAGES];
+    uint32_t group;
+} crdfft_radix_plan;
+
+typedef struct crdfft_many_info
+{
+    uint64_t transforms;
+    uint64_t lengths;       /* distinct lengths */
+    uint64_t tiles;
+    uint64_t points;        /* complex points over all transforms */
+    uint32_t workgroups;    /* persistent grid */
+    double imbalance;       /* max over workgroups of planned cost / mean */
+} crdfft_many_info;
+
+/* The radix plans the autotuner chooses among for a 2/3/5/7-smooth n (INVALID_SIZE otherwise); candidate 0 is the
+ * planner's default. Writes at most `capacity`, sets *count to the number that exist. */
+CRDFFT_API crdfft_status crdfft_radix_candidates(uint64_t n, crdfft_radix_plan* out, uint32_t capacity,
+                                                 uint32_t* count);
+/* `count` transforms, transform t of lengths[t] at element offset offsets[t] of both in and out (NULL: packed in
+ * order). choices: radix plans per length (a length without one gets candidate 0). */
+CRDFFT_API crdfft_status crdfft_many_plan_create(crdfft_many_plan* plan, const uint64_t* lengths,
+                                                 const uint64_t* offsets, uint64_t count,
+                                                 const crdfft_radix_plan* choices, uint64_t nchoices, uint32_t flags);
+CRDFFT_API crdfft_status crdfft_many_plan_destroy(crdfft_many_plan plan);
+CRDFFT_API crdfft_status crdfft_many_plan_get_info(crdfft_many_plan plan, crdfft_many_info* info);
+/* One persistent kernel launch on `stream`. in == out is allowed. */
+CRDFFT_API crdfft_status crdfft_many_execute(crdfft_many_plan plan, const void* in, void* out, int32_t direction,
+                                             void* stream);
+/* Captures one crdfft_many_execute on `stream` into a hipGraph the plan owns (replacing an earlier one), and
+ * instantiates it; crdfft_many_graph_launch replays it on any stream. */
+CRDFFT_API crdfft_status crdfft_many_graph_capture(crdfft_many_plan plan, const void* in, void* out,
+                                                   int32_t direction, void* stream);
+CRDFFT_API crdfft_status crdfft_many_graph_launch(crdfft_many_plan plan, void* stream);
+
+/* One autotuner measurement: a length, the candidate index, its radix plan and the measured time per transform. */
+typedef struct crdfft_tune_record
+{
+    crdfft_radix_plan plan;
+    uint32_t candidate;
+    uint32_t transforms;    /* transforms in the timed launch */
+    double us_per_transform;
+} crdfft_tune_record;
+
+/* The autotuner of the mixed-size path: for every distinct length of `lengths`, times every candidate radix plan
+ * (crdfft_radix_candidates) in the persistent kernel on a synthetic batch of at least 2^20 points (median of `reps`
+ * launches, hipEvents) and writes the fastest plan per length to choices (capacity >= distinct lengths). Every
+ * measurement is appended to records when records != NULL (at most record_capacity). Synchronous. */
+CRDFFT_API crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint32_t reps,
+                                              crdfft_radix_plan* choices, uint64_t capacity, uint64_t* nchoices,
+                                              crdfft_tune_record* records, uint64_t record_capacity,
+                                              uint64_t* nrecords);
+
+/* ---- the autotuner of single-length plans and the learned cost model ----
+ * One timed plan: the shape, the resolved knobs and the median time of one execute. */
+typedef struct crdfft_plan_record
+{
+    uint64_t n;           /* transform length (the volume for N-D) */
+    uint64_t batch;
+    int32_t type;
+    int32_t precision;
+    int32_t algorithm;
+    uint32_t ndim;        /* 1 for 1D; N-D: dims[0..ndim) */
+    uint64_t dims[3];
+    uint64_t stride;      /* 1 = contiguous transforms */
+    crdfft_tuning tuning; /* N-D: hints bit 2 = CRDFFT_ND_TRANSPOSE_ALL */
+    double ms;
+} crdfft_plan_record;
+
+#define CRDFFT_FEATURES 32
+/* The candidate knob settings of a power-of-two plan (candidate 0: the planner's default). */
+CRDFFT_API crdfft_status crdfft_plan_candidates(uint64_t n, uint64_t batch, int32_t type, crdfft_tuning* out,
+                                                uint32_t capacity, uint32_t* count);
+/* Times one plan (median of `reps` executes after a warm one, hipEvents, device buffers of the plan's shape). */
+CRDFFT_API crdfft_status crdfft_time_plan(uint64_t n, uint64_t batch, int32_t type, const crdfft_tuning* tuning,
+                                          uint32_t reps, crdfft_plan_record* record);
+/* Times an N-D plan (flags: CRDFFT_ND_TRANSPOSE_ALL or 0). */
+CRDFFT_API crdfft_status crdfft_time_nd(const uint64_t* dims, uint32_t ndim, uint64_t batch, uint32_t flags,
+                                        uint32_t reps, crdfft_plan_record* record);
+/* Exhaustive autotuning: times every candidate, returns the fastest; records every measurement. */
+CRDFFT_API crdfft_status crdfft_autotune_plan(uint64_t n, uint64_t batch, int32_t type, uint32_t reps,
+                                              crdfft_tuning* best, crdfft_plan_record* records,
+                                              uint32_t record_capacity, uint32_t* nrecords);
+/* CRDFFT_FEATURES floats per record: the factorization of n (exponents of 2, 3, 5, 7), batch, type, precision,
+ * stride, algorithm and knobs, N-D dims, all scaled to about [0, 1]. */
+CRDFFT_API crdfft_status crdfft_plan_features(const crdfft_plan_record* records, uint64_t count, float* out);
+
+/* A 3-layer perceptron (inputs -> hidden -> hidden -> 1, ReLU, FP32) whose inference and training (forward,
+ * backward, Adam) run in this library's own HIP kernels. Parameters, flat: W1[hidden][inputs], b1[hidden],
+ * W2[hidden][hidden], b2[hidden], w3[hidden], b3. Host arrays in and out. */
+typedef struct crdfft_mlp_s* crdfft_mlp;
+CRDFFT_API crdfft_status crdfft_mlp_create(crdfft_mlp* mlp, uint32_t inputs, uint32_t hidden, uint64_t seed);
+CRDFFT_API crdfft_status crdfft_mlp_destroy(crdfft_mlp mlp);
+CRDFFT_API crdfft_status crdfft_mlp_param_count(crdfft_mlp mlp, uint64_t* count);
+CRDFFT_API crdfft_status crdfft_mlp_get_params(crdfft_mlp mlp, float* params);
+CRDFFT_API crdfft_status crdfft_mlp_set_params(crdfft_mlp mlp, const float* params);
+/* y[i] = f(x[i]) for n rows of `inputs` floats. */
+CRDFFT_API crdfft_status crdfft_mlp_predict(crdfft_mlp mlp, const float* x, uint64_t n, float* y);
+/* Mean squared error over n rows and its gradient with respect to every parameter (flat layout). */
+CRDFFT_API crdfft_status crdfft_mlp_loss_grad(crdfft_mlp mlp, const float* x, const float* t, uint64_t n,
+                                              float* loss, float* grad);
+/* Adam on the mean squared error: `epochs` passes over n rows in seeded shuffled minibatches of `batch`. */
+CRDFFT_API crdfft_status crdfft_mlp_train(crdfft_mlp mlp, const float* x, const float* t, uint64_t n,
+                                          uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                                          float* final_loss);
 
 CRDFFT_API const char* crdfft_status_string(crdfft_status status);
 CRDFFT_API int32_t crdfft_last_hip_error(void);

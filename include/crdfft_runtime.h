@@ -35,6 +35,8 @@ CRDFFT_API crdfft_status crdfft_free(void* ptr);
 CRDFFT_API crdfft_status crdfft_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 CRDFFT_API crdfft_status crdfft_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 CRDFFT_API crdfft_status crdfft_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
+/* Streaming device copy with 16-byte accesses (bytes a multiple of 16): the achievable-bandwidth probe. */
+CRDFFT_API crdfft_status crdfft_stream_copy(void* dst, const void* src, uint64_t bytes, void* stream);
 /* Fills `bytes` with the byte `value` by a kernel on `stream` (no hipMemset node, safe under graph capture). */
 CRDFFT_API crdfft_status crdfft_fill_bytes(void* dst, int32_t value, uint64_t bytes, void* stream);
 

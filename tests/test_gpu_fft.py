@@ -266,3 +266,48 @@ def test_cpp_interface_runs(gpu, tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 10007, 7 * 11 * 13 * 17])
+def test_bluestein_any_length(gpu, po, n):
+    """BluesteinPlan (bluestein.hpp): forward unnormalized, inverse normalized, vs FP64 and the FP32 restatement."""
+    batch = 3
+    x = po.lcg_complex(n * batch, 7 * n + 1).reshape(batch, n)
+    plan = gpu.BluesteinPlan(n, batch)
+    assert plan.conv_size() >= 2 * n - 1
+    src = gpu.DeviceBuffer.from_array(x)
+    dst = gpu.DeviceBuffer(x.nbytes)
+    for inverse in (False, True):
+        plan.execute(src, dst, gpu.INVERSE if inverse else gpu.FORWARD)
+        gpu.synchronize()
+        y = dst.download(x.shape, np.complex64)
+        ref = po.fft64_of_f32(x, inverse=inverse)
+        if inverse:
+            ref = ref / n
+        assert po.rel_l2(y, ref) <= 2 * bound(max(n, 2)), f"n={n} inverse={inverse}"
+        r32 = np.stack([po.bluestein32(row, inverse) for row in x])
+        assert po.rel_l2(y, r32) <= 4 * bound(max(n, 2))
+    # in place
+    plan.execute(src, src, gpu.FORWARD)
+    gpu.synchronize()
+    assert po.rel_l2(src.download(x.shape, np.complex64), po.fft64_of_f32(x)) <= 2 * bound(max(n, 2))
+
+
+@pytest.mark.parametrize("dims", [(8, 16), (64, 64, 64), (3, 5, 7), (16, 1, 32), (5, 64), (64, 5), (2, 8192, 4),
+                                  (256, 256), (1, 1), (12,)])
+def test_nd_plan(gpu, po, dims):
+    """NdFftPlan (nd_fft.hpp): forward = N-D DFT, inverse = 1/prod(dims) IDFT, against numpy's FP64 N-D DFT."""
+    batch = 2
+    total = int(np.prod(dims))
+    x = po.lcg_complex(total * batch, total).reshape((batch,) + tuple(dims))
+    plan = gpu.NdFftPlan(dims, batch)
+    buf = gpu.DeviceBuffer.from_array(x)
+    axes = tuple(range(1, len(dims) + 1))
+    for inverse in (False, True):
+        buf.upload(x)
+        plan.execute(buf, gpu.INVERSE if inverse else gpu.FORWARD)
+        gpu.synchronize()
+        y = buf.download(x.shape, np.complex64)
+        xd = x.astype(np.complex128)
+        ref = np.fft.ifftn(xd, axes=axes) if inverse else np.fft.fftn(xd, axes=axes)
+        assert po.rel_l2(y, ref) <= 3 * bound(max(total, 2)), f"dims={dims} inverse={inverse}"
