@@ -89,6 +89,38 @@ class DeviceProps(ctypes.Structure):
     ]
 
 
+MAX_STAGES = 16
+FEATURES = 32
+
+
+class RadixPlan(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("nstages", ctypes.c_uint32), ("radix", ctypes.c_uint32 * MAX_STAGES),
+                ("group", ctypes.c_uint32)]
+
+    def radices(self):
+        return [int(self.radix[i]) for i in range(self.nstages)]
+
+    def as_dict(self):
+        return {"n": int(self.n), "radices": self.radices(), "group": int(self.group)}
+
+
+class ManyInfo(ctypes.Structure):
+    _fields_ = [("transforms", ctypes.c_uint64), ("lengths", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
+                ("points", ctypes.c_uint64), ("workgroups", ctypes.c_uint32), ("imbalance", ctypes.c_double)]
+
+
+class TuneRecord(ctypes.Structure):
+    _fields_ = [("plan", RadixPlan), ("candidate", ctypes.c_uint32), ("transforms", ctypes.c_uint32),
+                ("us_per_transform", ctypes.c_double)]
+
+
+class PlanRecord(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("batch", ctypes.c_uint64), ("type", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("algorithm", ctypes.c_int32), ("ndim", ctypes.c_uint32),
+                ("dims", ctypes.c_uint64 * 3), ("stride", ctypes.c_uint64), ("tuning", Tuning),
+                ("ms", ctypes.c_double)]
+
+
 _lib = None
 
 _vp = ctypes.c_void_p
@@ -132,6 +164,35 @@ _SIGS = {
     "crdfft_event_record": (_i32, [_vp, _vp]),
     "crdfft_event_synchronize": (_i32, [_vp]),
     "crdfft_event_elapsed_ms": (_i32, [ctypes.POINTER(ctypes.c_float), _vp, _vp]),
+    # mixed-size persistent launch and its autotuner
+    "crdfft_radix_candidates": (_i32, [_u64, ctypes.POINTER(RadixPlan), _u32, ctypes.POINTER(_u32)]),
+    "crdfft_many_plan_create": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(_u64), ctypes.POINTER(_u64), _u64,
+                                       ctypes.POINTER(RadixPlan), _u64, _u32]),
+    "crdfft_many_plan_destroy": (_i32, [_vp]),
+    "crdfft_many_plan_get_info": (_i32, [_vp, ctypes.POINTER(ManyInfo)]),
+    "crdfft_many_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "crdfft_many_graph_capture": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "crdfft_many_graph_launch": (_i32, [_vp, _vp]),
+    "crdfft_many_autotune": (_i32, [ctypes.POINTER(_u64), _u64, _u32, ctypes.POINTER(RadixPlan), _u64,
+                                    ctypes.POINTER(_u64), ctypes.POINTER(TuneRecord), _u64, ctypes.POINTER(_u64)]),
+    # single-length autotuner, records and features
+    "crdfft_plan_candidates": (_i32, [_u64, _u64, _i32, ctypes.POINTER(Tuning), _u32, ctypes.POINTER(_u32)]),
+    "crdfft_time_plan": (_i32, [_u64, _u64, _i32, ctypes.POINTER(Tuning), _u32, ctypes.POINTER(PlanRecord)]),
+    "crdfft_time_nd": (_i32, [ctypes.POINTER(_u64), _u32, _u64, _u32, _u32, ctypes.POINTER(PlanRecord)]),
+    "crdfft_autotune_plan": (_i32, [_u64, _u64, _i32, _u32, ctypes.POINTER(Tuning), ctypes.POINTER(PlanRecord),
+                                    _u32, ctypes.POINTER(_u32)]),
+    "crdfft_plan_features": (_i32, [ctypes.POINTER(PlanRecord), _u64, ctypes.POINTER(ctypes.c_float)]),
+    # the cost model
+    "crdfft_mlp_create": (_i32, [ctypes.POINTER(_vp), _u32, _u32, _u64]),
+    "crdfft_mlp_destroy": (_i32, [_vp]),
+    "crdfft_mlp_param_count": (_i32, [_vp, ctypes.POINTER(_u64)]),
+    "crdfft_mlp_get_params": (_i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "crdfft_mlp_set_params": (_i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "crdfft_mlp_predict": (_i32, [_vp, ctypes.POINTER(ctypes.c_float), _u64, ctypes.POINTER(ctypes.c_float)]),
+    "crdfft_mlp_loss_grad": (_i32, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _u64,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "crdfft_mlp_train": (_i32, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _u64, _u32,
+                                _u32, ctypes.c_float, _u64, ctypes.POINTER(ctypes.c_float)]),
 }
 
 

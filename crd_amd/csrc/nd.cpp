@@ -97,7 +97,7 @@ crdfft_status line_plan(uint64_t d, uint64_t lines, crdfft_plan* out, bool* blue
     return crdfft_plan_create(out, d, lines, d, d, CRDFFT_C2C, CRDFFT_FP32, CRDFFT_FLAG_NO_SELF_CHECK);
 }
 
-crdfft_status build(crdfft_nd_plan_s* p, const uint64_t* dims, uint32_t ndim, uint64_t batch)
+crdfft_status build(crdfft_nd_plan_s* p, const uint64_t* dims, uint32_t ndim, uint64_t batch, uint32_t flags)
 {
     p->batch = batch;
     p->axes.resize(ndim);
@@ -132,7 +132,8 @@ crdfft_status build(crdfft_nd_plan_s* p, const uint64_t* dims, uint32_t ndim, ui
                 return st;
             }
         }
-        else if (crdfft::is_pow2(a.d) && crdfft::element_major_shape(a.d, a.inner, &a.e, &a.c))
+        else if ((flags & CRDFFT_ND_TRANSPOSE_ALL) == 0 && crdfft::is_pow2(a.d) &&
+                 crdfft::element_major_shape(a.d, a.inner, &a.e, &a.c))
         {
             a.kind = kColumns;
             const uint64_t entries = std::max<uint64_t>(1, crdfft::stage_twiddle_entries(a.d, a.e));
@@ -165,7 +166,6 @@ extern "C" {
 crdfft_status crdfft_nd_plan_create(crdfft_nd_plan* plan, const uint64_t* dims, uint32_t ndim, uint64_t batch,
                                     uint32_t flags)
 {
-    (void)flags;
     if (plan == nullptr || dims == nullptr || ndim == 0 || batch == 0)
     {
         return CRDFFT_INVALID_ARGUMENT;
@@ -188,13 +188,90 @@ crdfft_status crdfft_nd_plan_create(crdfft_nd_plan* plan, const uint64_t* dims, 
     {
         return CRDFFT_OUT_OF_MEMORY;
     }
-    const crdfft_status st = build(p, dims, ndim, batch);
+    const crdfft_status st = build(p, dims, ndim, batch, flags);
     if (st != CRDFFT_OK)
     {
         delete p;
         return st;
     }
     *plan = p;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_time_nd(const uint64_t* dims, uint32_t ndim, uint64_t batch, uint32_t flags, uint32_t reps,
+                            crdfft_plan_record* record)
+{
+    if (record == nullptr || dims == nullptr || ndim == 0 || ndim > 3)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    crdfft_nd_plan plan = nullptr;
+    crdfft_status st = crdfft_nd_plan_create(&plan, dims, ndim, batch, flags);
+    if (st != CRDFFT_OK)
+    {
+        return st;
+    }
+    void* data = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const uint64_t bytes = plan->total * batch * sizeof(crdfft::cfloat);
+    std::vector<float> ms(std::max<uint32_t>(reps, 1));
+    if (hipMalloc(&data, bytes) != hipSuccess)
+    {
+        crdfft_nd_plan_destroy(plan);
+        return CRDFFT_OUT_OF_MEMORY;
+    }
+    if (hipMemset(data, 0, bytes) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    {
+        st = CRDFFT_HIP_ERROR;
+    }
+    if (st == CRDFFT_OK)
+    {
+        st = crdfft_nd_execute(plan, data, CRDFFT_FORWARD, s);
+    }
+    for (size_t r = 0; r < ms.size() && st == CRDFFT_OK; ++r)
+    {
+        if (hipEventRecord(e0, s) != hipSuccess)
+        {
+            st = CRDFFT_HIP_ERROR;
+            break;
+        }
+        st = crdfft_nd_execute(plan, data, CRDFFT_FORWARD, s);
+        if (st == CRDFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms[r], e0, e1) != hipSuccess))
+        {
+            st = CRDFFT_HIP_ERROR;
+        }
+    }
+    if (e0 != nullptr)
+        (void)hipEventDestroy(e0);
+    if (e1 != nullptr)
+        (void)hipEventDestroy(e1);
+    if (s != nullptr)
+        (void)hipStreamDestroy(s);
+    (void)hipFree(data);
+    const uint64_t total = plan->total;
+    crdfft_nd_plan_destroy(plan);
+    if (st != CRDFFT_OK)
+    {
+        return st;
+    }
+    std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
+    *record = crdfft_plan_record{};
+    record->n = total;
+    record->batch = batch;
+    record->type = CRDFFT_C2C;
+    record->precision = CRDFFT_FP32;
+    record->algorithm = CRDFFT_ALGO_FOUR_STEP;
+    record->ndim = ndim;
+    for (uint32_t i = 0; i < ndim; ++i)
+    {
+        record->dims[i] = dims[i];
+    }
+    record->stride = ndim > 1 ? dims[ndim - 1] : 1;
+    record->tuning.hints = (flags & CRDFFT_ND_TRANSPOSE_ALL) ? 4u : 0u;
+    record->ms = ms[ms.size() / 2];
     return CRDFFT_OK;
 }
 

@@ -17,6 +17,7 @@
 #include "../../include/crdfft.h"
 #include "../../include/crdfft_runtime.h"
 #include "kernels.hpp"
+#include "mfma.hpp"
 #include "planner.hpp"
 #include "real.hpp"
 
@@ -96,6 +97,7 @@ struct crdfft_plan_s
     DeviceBuffer wref;      // W_cn^k, k < cn/2, for the radix-2 reference (built on first use)
     DeviceBuffer workspace; // four-step intermediate, chunk * complex_n
     DeviceBuffer tw_em;     // stage table of the element-major kernel (built on first use)
+    DeviceBuffer split_fwd, split_inv, tw256; // split-precision MFMA passes: operand splits, W_256
     // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
     crdfft_plan inner = nullptr;
     DeviceBuffer chirp, bhat, bwork;
@@ -186,6 +188,23 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
 {
     const crdfft::PlanShape& sh = p->shape;
     const uint64_t cn = sh.complex_n;
+    if (sh.precision != CRDFFT_FP32)
+    {
+        const uint64_t fl = crdfft::split_table_floats();
+        CRDFFT_TRY(p->split_fwd.alloc(fl * sizeof(float)));
+        CRDFFT_TRY(p->split_inv.alloc(fl * sizeof(float)));
+        CRDFFT_TRY(crdfft::build_split_tables(p->split_fwd.as<float>(), sh.precision, false, s));
+        CRDFFT_TRY(crdfft::build_split_tables(p->split_inv.as<float>(), sh.precision, true, s));
+        CRDFFT_TRY(p->tw256.alloc(256 * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_roots(p->tw256.as<cfloat>(), 256, 1, 256, s));
+        p->hshift = 8;
+        CRDFFT_TRY(p->thi.alloc(256 * sizeof(cfloat)));
+        CRDFFT_TRY(p->tlo.alloc(256 * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), 256, 256, cn, s));
+        CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), 256, 1, cn, s));
+        CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
+        return CRDFFT_OK;
+    }
     if (sh.algorithm == CRDFFT_ALGO_SINGLE)
     {
         const uint64_t entries = crdfft::stage_twiddle_entries(cn, sh.e);
@@ -294,7 +313,25 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
     }
     if (sh.precision != CRDFFT_FP32)
     {
-        return CRDFFT_UNSUPPORTED;
+        // split-precision MFMA four-step of 256 x 256: column pass into the workspace, row pass out
+        const cfloat* x = static_cast<const cfloat*>(in);
+        cfloat* y = static_cast<cfloat*>(out);
+        const float* tab = inverse ? p->split_inv.as<float>() : p->split_fwd.as<float>();
+        for (uint64_t c0 = 0; c0 < sh.batch; c0 += sh.chunk)
+        {
+            const uint64_t cnt = std::min(sh.chunk, sh.batch - c0);
+            crdfft::SplitArgs a{x + c0 * p->idist, p->workspace.as<cfloat>(), tab, p->tw256.as<cfloat>(),
+                                p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, p->idist, sh.complex_n};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_split_pass(0, sh.precision, inverse, a, s));
+            CRDFFT_TRY(prof.after(0, sh.complex_n * cnt));
+            crdfft::SplitArgs b{p->workspace.as<cfloat>(), y + c0 * p->odist, tab, p->tw256.as<cfloat>(),
+                                p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, sh.complex_n, p->odist};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_split_pass(1, sh.precision, inverse, b, s));
+            CRDFFT_TRY(prof.after(1, sh.complex_n * cnt));
+        }
+        return CRDFFT_OK;
     }
     if (sh.algorithm == CRDFFT_ALGO_BLUESTEIN)
     {
@@ -663,7 +700,7 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
     }
     crdfft::shape_to_info(p->shape, info);
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
-                        p->tstep.bytes + p->wreal.bytes;
+                        p->tstep.bytes + p->wreal.bytes + p->split_fwd.bytes + p->split_inv.bytes + p->tw256.bytes;
     return CRDFFT_OK;
 }
 

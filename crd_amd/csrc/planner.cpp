@@ -129,6 +129,28 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
     {
         return CRDFFT_INVALID_SIZE;
     }
+    if (precision != CRDFFT_FP32)
+    {
+        // the split-precision MFMA passes exist for the four-step of 256 x 256 (complex n = 2^16)
+        if (type != CRDFFT_C2C || cn != 65536)
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+        s.algorithm = CRDFFT_ALGO_FOUR_STEP;
+        s.n1 = 256;
+        s.n2 = 256;
+        s.e1 = s.e2 = 16;
+        s.tile1 = s.tile2 = 16;
+        s.chunk = std::max<uint64_t>(1, kChunkBytes / (cn * 8));
+        if (tuning != nullptr && tuning->chunk != 0)
+        {
+            s.chunk = tuning->chunk;
+        }
+        s.chunk = std::min<uint64_t>(s.chunk, std::max<uint64_t>(batch, 1));
+        s.workspace_bytes = s.chunk * cn * 8;
+        *out = s;
+        return CRDFFT_OK;
+    }
     if (cn == 1)
     {
         s.algorithm = CRDFFT_ALGO_COPY;

@@ -12,60 +12,11 @@ from . import _native as N
 from ._native import check
 from .fft import _ptr, _stream
 
-MAX_STAGES = 16
-
-
-class RadixPlan(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_uint64), ("nstages", ctypes.c_uint32), ("radix", ctypes.c_uint32 * MAX_STAGES),
-                ("group", ctypes.c_uint32)]
-
-    def radices(self):
-        return [int(self.radix[i]) for i in range(self.nstages)]
-
-    def as_dict(self):
-        return {"n": int(self.n), "radices": self.radices(), "group": int(self.group)}
-
-
-class ManyInfo(ctypes.Structure):
-    _fields_ = [("transforms", ctypes.c_uint64), ("lengths", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
-                ("points", ctypes.c_uint64), ("workgroups", ctypes.c_uint32), ("imbalance", ctypes.c_double)]
-
-
-class TuneRecord(ctypes.Structure):
-    _fields_ = [("plan", RadixPlan), ("candidate", ctypes.c_uint32), ("transforms", ctypes.c_uint32),
-                ("us_per_transform", ctypes.c_double)]
-
-
-def _bind():
-    L = N.lib()
-    vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32
-    sigs = {
-        "crdfft_radix_candidates": (i32, [u64, ctypes.POINTER(RadixPlan), u32, ctypes.POINTER(u32)]),
-        "crdfft_many_plan_create": (i32, [ctypes.POINTER(vp), ctypes.POINTER(u64), ctypes.POINTER(u64), u64,
-                                          ctypes.POINTER(RadixPlan), u64, u32]),
-        "crdfft_many_plan_destroy": (i32, [vp]),
-        "crdfft_many_plan_get_info": (i32, [vp, ctypes.POINTER(ManyInfo)]),
-        "crdfft_many_execute": (i32, [vp, vp, vp, i32, vp]),
-        "crdfft_many_graph_capture": (i32, [vp, vp, vp, i32, vp]),
-        "crdfft_many_graph_launch": (i32, [vp, vp]),
-        "crdfft_many_autotune": (i32, [ctypes.POINTER(u64), u64, u32, ctypes.POINTER(RadixPlan), u64,
-                                       ctypes.POINTER(u64), ctypes.POINTER(TuneRecord), u64, ctypes.POINTER(u64)]),
-    }
-    for name, (res, args) in sigs.items():
-        fn = getattr(L, name)
-        fn.restype = res
-        fn.argtypes = args
-    return L
-
-
-_L = None
+from ._native import ManyInfo, RadixPlan, TuneRecord  # noqa: F401
 
 
 def lib():
-    global _L
-    if _L is None:
-        _L = _bind()
-    return _L
+    return N.lib()
 
 
 def radix_candidates(n):

@@ -75,6 +75,12 @@ void oracle_bluestein32(float* data, size_t n, int inverse);
 /* Hardware threads the batch functions use for threads <= 0. */
 int oracle_hardware_threads(void);
 
+/* ---- FP64 restatement of the plan cost model (mlp.cpp): 3-layer ReLU perceptron, flat parameters
+ * W1[h][in], b1[h], W2[h][h], b2[h], w3[h], b3; pinned by central finite differences. ---- */
+void oracle_mlp_forward(const double* params, uint32_t inputs, uint32_t hidden, const double* x, uint64_t n, double* y);
+double oracle_mlp_loss_grad(const double* params, uint32_t inputs, uint32_t hidden, const double* x, const double* t,
+                            uint64_t n, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,8 @@ def lib():
             "oracle_irfft32_batch": (None, [vp, vp, sz, sz, i]),
             "oracle_bluestein32": (None, [vp, sz, i]),
             "oracle_hardware_threads": (i, []),
+            "oracle_mlp_forward": (None, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp]),
+            "oracle_mlp_loss_grad": (ctypes.c_double, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -207,3 +209,23 @@ def max_rel(y, ref):
     y = np.asarray(y, dtype=np.complex128)
     ref = np.asarray(ref, dtype=np.complex128)
     return float(np.max(np.abs(y - ref)) / (1.0 + np.max(np.abs(ref)))) if ref.size else 0.0
+
+
+def mlp_forward(params, inputs, hidden, x):
+    """FP64 forward of the cost model (oracle/mlp.cpp)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[0]
+    y = np.empty(n, dtype=np.float64)
+    lib().oracle_mlp_forward(_p(p), inputs, hidden, _p(x), n, _p(y))
+    return y
+
+
+def mlp_loss_grad(params, inputs, hidden, x, t):
+    """FP64 mean squared error and its gradient (flat, the parameters' layout)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.empty_like(p)
+    loss = lib().oracle_mlp_loss_grad(_p(p), inputs, hidden, _p(x), _p(t), x.shape[0], _p(g))
+    return loss, g

@@ -1,0 +1,82 @@
+"""The learned plan cost model on the device: forward, gradients and training checked against the oracle's FP64
+restatement (oracle/mlp.cpp), and what ranking with it gains over not using it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_and_gradient_match_fp64_oracle(gpu, po):
+    from crd_amd import costmodel
+
+    m = costmodel.Mlp(seed=3)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1, (37, costmodel.FEATURES)).astype(np.float32)
+    t = rng.standard_normal(37).astype(np.float32)
+    p = m.params()
+    y = m.predict(x)
+    y64 = po.mlp_forward(p.astype(np.float64), costmodel.FEATURES, costmodel.HIDDEN, x.astype(np.float64))
+    assert np.max(np.abs(y - y64)) <= 1e-5 * max(1.0, np.max(np.abs(y64)))
+    loss, g = m.loss_grad(x, t)
+    loss64, g64 = po.mlp_loss_grad(p.astype(np.float64), costmodel.FEATURES, costmodel.HIDDEN,
+                                   x.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - loss64) <= 1e-5 * max(1.0, loss64)
+    assert rel(g.astype(np.float64), g64) <= 1e-5
+    # every tensor of the flat layout, not only the norm of the whole
+    h, f = costmodel.HIDDEN, costmodel.FEATURES
+    cuts = np.cumsum([h * f, h, h * h, h, h, 1])
+    for a, b in zip(np.split(g.astype(np.float64), cuts[:-1]), np.split(g64, cuts[:-1])):
+        assert rel(a, b) <= 1e-4
+
+
+def test_training_is_deterministic_and_fits(gpu):
+    from crd_amd import costmodel
+
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0, 1, (2048, costmodel.FEATURES)).astype(np.float32)
+    t = (np.sin(3 * x[:, 0]) + x[:, 1] * x[:, 2] - 0.5 * x[:, 3]).astype(np.float32)
+    a, b = costmodel.Mlp(seed=2), costmodel.Mlp(seed=2)
+    before, _ = a.loss_grad(x, t)
+    la = a.train(x, t, epochs=60, batch=128, lr=3e-3, seed=9)
+    lb = b.train(x, t, epochs=60, batch=128, lr=3e-3, seed=9)
+    assert np.array_equal(a.params().view(np.uint32), b.params().view(np.uint32))
+    assert la == lb
+    assert la < 0.05 * before
+
+
+def test_one_adam_step_matches_oracle_gradient_step(gpu, po):
+    """A single Adam step moves each parameter by lr * sign(g) (first step, bias-corrected) for the oracle's g."""
+    from crd_amd import costmodel
+
+    m = costmodel.Mlp(seed=4)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0, 1, (64, costmodel.FEATURES)).astype(np.float32)
+    t = rng.standard_normal(64).astype(np.float32)
+    p0 = m.params().astype(np.float64)
+    _, g64 = po.mlp_loss_grad(p0, costmodel.FEATURES, costmodel.HIDDEN, x.astype(np.float64), t.astype(np.float64))
+    lr = 1e-3
+    m.train(x, t, epochs=1, batch=64, lr=lr, seed=1)
+    step = m.params().astype(np.float64) - p0
+    big = np.abs(g64) > 1e-4  # where the sign is unambiguous
+    assert np.allclose(step[big], -lr * np.sign(g64[big]), rtol=1e-3, atol=1e-7)
+
+
+def test_ranking_gains_over_no_model(gpu):
+    """Trained on timed candidates of seeded shapes, the model's top-2 picks (timing 2 candidates per case) must
+    land closer to the exhaustive best than 2 random candidates do, on held-out shapes."""
+    from crd_amd import costmodel
+
+    train = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(60, seed=11, lo=8, hi=20)]
+    held = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(40, seed=12, lo=8, hi=20)]
+    recs = [r for rs in train for r in rs]
+    x, y = costmodel.features(recs), costmodel.target(recs)
+    model = costmodel.Mlp(seed=1)
+    model.train(x, (y - y.mean()) / (y.std() + 1e-6), epochs=300, batch=128, lr=2e-3, seed=7)
+    rep = costmodel.evaluate(model, held, k_values=(2,))
+    print(rep)
+    assert rep["top2"]["mean_regret"] <= rep["top2"]["random_mean_regret"]
+    assert rep["top2"]["timed_fraction"] < 0.75

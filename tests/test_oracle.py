@@ -156,3 +156,25 @@ def test_real32_against_fp64(po, n):
     assert po.rel_l2(X, ref) <= 2e-6 * np.log2(n)
     back = po.irfft32(X, n)
     assert np.max(np.abs(back - x)) <= 2e-6 * np.log2(n) * np.max(np.abs(x))
+
+
+def test_mlp_gradient_matches_finite_differences(po):
+    """The oracle's backpropagation against central differences of its own forward (FP64)."""
+    inputs, hidden, n = 7, 5, 11
+    rng = np.random.default_rng(1)
+    count = hidden * inputs + hidden + hidden * hidden + hidden + hidden + 1
+    p = rng.standard_normal(count) * 0.5
+    x = rng.standard_normal((n, inputs))
+    t = rng.standard_normal(n)
+    loss, g = po.mlp_loss_grad(p, inputs, hidden, x, t)
+    y = po.mlp_forward(p, inputs, hidden, x)
+    assert abs(loss - np.mean((y - t) ** 2)) < 1e-12
+    h = 1e-6
+    num = np.empty_like(p)
+    for i in range(count):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        num[i] = (np.mean((po.mlp_forward(pp, inputs, hidden, x) - t) ** 2)
+                  - np.mean((po.mlp_forward(pm, inputs, hidden, x) - t) ** 2)) / (2 * h)
+    assert np.max(np.abs(num - g)) <= 1e-6 * max(1.0, np.max(np.abs(g)))
