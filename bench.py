@@ -106,6 +106,106 @@ def cpu_baseline(seconds_per_shape, threads):
     }
 
 
+def bench_mixed(stream, Event):
+    """20,000 C2C transforms of 7-smooth lengths in [64, 16384] (log-uniform lengths, Poisson(8) batches, seeded):
+    the autotuned persistent launch replayed from a hipGraph, against the reference's method (one plan per length
+    and batch group, Bluestein for every length that is not a power of two)."""
+    import crd_amd
+    from crd_amd import many, workloads
+
+    lengths = workloads.mixed_batch()
+    total = int(lengths.sum())
+    x = workloads.uniform_complex(total, 20000)
+    src = crd_amd.DeviceBuffer.from_array(x)
+    dst = crd_amd.DeviceBuffer(x.nbytes)
+    t0 = time.time()
+    choices, records = many.autotune(lengths, reps=3)
+    tune_s = time.time() - t0
+    out = {"transforms": int(len(lengths)), "lengths": int(len(set(lengths.tolist()))), "points": total,
+           "algorithmic_bytes": 16 * total, "autotune_seconds": round(tune_s, 2), "autotune_timings": len(records)}
+    for name, ch in (("tuned", choices), ("default_radix_plans", None)):
+        plan = many.ManyPlan(lengths, choices=ch)
+        plan.capture_graph(src, dst, crd_amd.FORWARD, stream)
+        ms = time_loop(lambda: plan.launch_graph(stream), stream, 20, 3, Event)
+        out[name] = {"ms": round(ms, 4), "ffts_per_s": round(len(lengths) / (ms * 1e-3), 1),
+                     "gbs": round(16.0 * total / (ms * 1e-3) / 1e9, 1)}
+        if name == "tuned":
+            info = plan.info()
+            out["tiles"] = int(info["tiles"])
+            out["imbalance"] = round(info["imbalance"], 3)
+            y = dst.download(x.shape, np.complex64)
+            errs_num, errs_den, pos = 0.0, 0.0, 0
+            rng = np.random.default_rng(1)
+            pick = set(rng.choice(len(lengths), 200, replace=False).tolist())
+            for i, n in enumerate(lengths.tolist()):
+                if i in pick:
+                    ref = np.fft.fft(x[pos:pos + n].astype(np.complex128))
+                    errs_num += float(np.sum(np.abs(y[pos:pos + n] - ref) ** 2))
+                    errs_den += float(np.sum(np.abs(ref) ** 2))
+                pos += n
+            out["tuned"]["rel_l2_vs_fp64_sample"] = (errs_num / errs_den) ** 0.5
+        del plan
+    # the reference's method: a plan per (length, group) in draw order, Bluestein off the powers of two
+    groups, start = [], 0
+    for i in range(1, len(lengths) + 1):
+        if i == len(lengths) or lengths[i] != lengths[start]:
+            groups.append((int(lengths[start]), start, i - start))
+            start = i
+    offs = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    plans = {}
+    calls = []
+    for n, first, cnt in groups:
+        key = (n, cnt)
+        if key not in plans:
+            plans[key] = crd_amd.FftPlan(n, cnt, self_check=False) if n & (n - 1) == 0 else \
+                crd_amd.BluesteinPlan(n, cnt)
+        calls.append((plans[key], src.address + 8 * int(offs[first]), dst.address + 8 * int(offs[first])))
+
+    def plain():
+        for p, a, b in calls:
+            p.execute(a, b, crd_amd.FORWARD, stream)
+
+    ms = time_loop(plain, stream, 3, 1, Event)
+    out["plain_per_length_bluestein"] = {"ms": round(ms, 4), "ffts_per_s": round(len(lengths) / (ms * 1e-3), 1),
+                                         "gbs": round(16.0 * total / (ms * 1e-3) / 1e9, 1), "plans": len(plans),
+                                         "launch_groups": len(calls)}
+    out["tuned_vs_plain_ffts_per_s"] = round(out["plain_per_length_bluestein"]["ms"] / out["tuned"]["ms"], 2)
+    return out
+
+
+def bench_split(stream, Event):
+    """n = 2^16, batch 2048, |x| log-uniform in [1e-4, 1e4] with uniform phase: plain FP32 against the split-BF16
+    and split-FP16 MFMA forms, time and rel-L2 against FP64 (numpy, float64) on a sample."""
+    import crd_amd
+
+    n, batch, seed_rows = 1 << 16, 2048, 64
+    rng = np.random.default_rng(4)
+    mag = 10.0 ** rng.uniform(-4, 4, (seed_rows, n))
+    x = (mag * np.exp(1j * rng.uniform(0, 2 * np.pi, (seed_rows, n)))).astype(np.complex64)
+    src = crd_amd.DeviceBuffer(8 * n * batch)
+    dst = crd_amd.DeviceBuffer(8 * n * batch)
+    src.upload(x)
+    for r in range(seed_rows, batch, seed_rows):  # the 64 seeded rows repeated over the batch
+        from crd_amd import _native as N
+
+        N.check(N.lib().crdfft_memcpy_d2d(src.address + 8 * n * r, src.ptr, 8 * n * seed_rows, stream.handle),
+                "crdfft_memcpy_d2d")
+    stream.synchronize()
+    ref = np.fft.fft(x[:4].astype(np.complex128), axis=1)
+    out = {"n": n, "batch": batch}
+    for prec in ("fp32", "split_bf16", "split_fp16"):
+        p = crd_amd.FftPlan(n, batch, precision=prec)
+        ms = time_loop(lambda: p.execute(src, dst, crd_amd.FORWARD, stream), stream, 5, 1, Event)
+        y = dst.download((4, n), np.complex64)
+        out[prec] = {"ms": round(ms, 4), "ffts_per_s": round(batch / (ms * 1e-3), 1),
+                     "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
+                     "rel_l2_vs_fp64": rel_l2(y.astype(np.complex128), ref)}
+        del p
+    out["split_bf16_vs_fp32_ffts_per_s"] = round(out["fp32"]["ms"] / out["split_bf16"]["ms"], 3)
+    out["split_fp16_vs_fp32_ffts_per_s"] = round(out["fp32"]["ms"] / out["split_fp16"]["ms"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +214,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the value (for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--inputs", choices=["normal", "device"], default="normal",
+                    help="normal: seeded N(0,1) uploaded from the host; device: a byte fill on the device (profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB streaming copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -138,7 +242,11 @@ def main():
         nbytes = 8 * n * batch
         xin = DeviceBuffer(nbytes)
         xout = DeviceBuffer(nbytes)
-        firsts.append(fill_normal(xin, 2 * n * batch, seed=1000 + idx))
+        if args.inputs == "normal":
+            firsts.append(fill_normal(xin, 2 * n * batch, seed=1000 + idx))
+        else:
+            xin.fill(0x3E)  # every float 0.18575... : finite, normal, no work skipped
+            firsts.append(None)
         plans.append(p)
         bufs.append((xin, xout))
 
@@ -222,9 +330,20 @@ def main():
         except (ValueError, KeyError):
             pass
 
+    if args.calibrate_copy:
+        from crd_amd import _native as N
+
+        cal = DeviceBuffer(1 << 30)
+        cal2 = DeviceBuffer(1 << 30)
+        N.check(N.lib().crdfft_stream_copy(cal2.ptr, cal.ptr, 1 << 30, stream.handle), "crdfft_stream_copy")
+        stream.synchronize()
+        del cal, cal2
+
     # accuracy of a sample vs FP64 (numpy's pocketfft in float64)
     acc_report = []
     for (n, batch), (xin, xout), first in zip(SHAPES, bufs, firsts):
+        if first is None:
+            continue
         m = min(len(first) // (2 * n), 4)
         x = first[: 2 * n * m].view(np.complex64).reshape(m, n)
         y = xout.download((m, n), np.complex64)
@@ -278,6 +397,8 @@ def main():
         ms = time_loop(rot, stream, 40, 4, Event)
         extras["c2c_n4096_b1024"] = {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
                                      "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
+        extras["mixed_size"] = bench_mixed(stream, Event)
+        extras["split_precision"] = bench_split(stream, Event)
         result["extras"] = extras
 
     if not args.no_cpu:
