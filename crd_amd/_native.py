@@ -16,8 +16,8 @@ OK, INVALID_SIZE, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, SELF_CHECK_FAILED,
 FORWARD, INVERSE = 0, 1
 C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
-ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN = 0, 1, 2, 3, 4
-FLAG_NONE, FLAG_NO_SELF_CHECK = 0, 1
+ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED = 0, 1, 2, 3, 4, 5
+FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
 
 
 class CrdfftError(RuntimeError):
@@ -64,6 +64,7 @@ class Tuning(ctypes.Structure):
         ("tile2", ctypes.c_int32),
         ("chunk", ctypes.c_uint64),
         ("hints", ctypes.c_uint32),
+        ("algorithm", ctypes.c_int32),
     ]
 
 

@@ -76,14 +76,14 @@ template <bool NT> __device__ __forceinline__ void stg(cf* p, cf v)
 
 // v[m] = src[m * stride] for m < E, with or without the non-temporal hint (a uniform branch).
 template <int E>
-__device__ __forceinline__ void load_strided(cf (&v)[E], const cf* src, uint64_t stride, bool nt)
+__device__ __forceinline__ void load_strided(cf (&v)[E], const cf* src, uint32_t stride, bool nt)
 {
     if (nt)
     {
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {
-            v[m] = ldg<true>(src + m * stride);
+            v[m] = ldg<true>(src + static_cast<uint32_t>(m) * stride);
         }
     }
     else
@@ -91,20 +91,20 @@ __device__ __forceinline__ void load_strided(cf (&v)[E], const cf* src, uint64_t
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {
-            v[m] = ldg<false>(src + m * stride);
+            v[m] = ldg<false>(src + static_cast<uint32_t>(m) * stride);
         }
     }
 }
 
 template <int E>
-__device__ __forceinline__ void store_strided(const cf (&v)[E], cf* dst, uint64_t stride, float scale, bool nt)
+__device__ __forceinline__ void store_strided(const cf (&v)[E], cf* dst, uint32_t stride, float scale, bool nt)
 {
     if (nt)
     {
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {
-            stg<true>(dst + m * stride, cf{v[m].x * scale, v[m].y * scale});
+            stg<true>(dst + static_cast<uint32_t>(m) * stride, cf{v[m].x * scale, v[m].y * scale});
         }
     }
     else
@@ -112,7 +112,82 @@ __device__ __forceinline__ void store_strided(const cf (&v)[E], cf* dst, uint64_
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {
-            stg<false>(dst + m * stride, cf{v[m].x * scale, v[m].y * scale});
+            stg<false>(dst + static_cast<uint32_t>(m) * stride, cf{v[m].x * scale, v[m].y * scale});
+        }
+    }
+}
+
+// Raw buffer addressing: the base address sits in a 128-bit resource in scalar registers and every access takes a
+// 32-bit byte offset per lane plus a scalar offset, so the E accesses of a thread share one address VGPR and cost
+// no vector address arithmetic (the per-m stride goes into the scalar offset). Offsets stay below 4 GiB from the
+// base, which the launchers guarantee by making one resource per matrix.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+
+// The builtins' own 8-byte vector type (a generic vector, converted as a whole: its elements are not accessed).
+using b64_t = decltype(__builtin_amdgcn_raw_buffer_load_b64(make_rsrc(nullptr), 0, 0, 0));
+static_assert(sizeof(b64_t) == 8, "64-bit buffer access");
+constexpr int kAuxNt = 2; // cache policy of the buffer instructions: non-temporal
+
+template <int AUX> __device__ __forceinline__ cf bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    const b64_t u = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(voff), static_cast<int>(soff), AUX);
+    const f2v f = __builtin_bit_cast(f2v, u);
+    return cf{f.x, f.y};
+}
+
+template <int AUX> __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, cf v)
+{
+    f2v f;
+    f.x = v.x;
+    f.y = v.y;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64_t, f), r, static_cast<int>(voff),
+                                          static_cast<int>(soff), AUX);
+}
+
+// v[m] = element (off + m * stride) of the resource's array, for m < E (offsets in elements).
+template <int E>
+__device__ __forceinline__ void buf_load_strided(cf (&v)[E], __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t stride,
+                                                 bool nt)
+{
+    if (nt)
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = bld<kAuxNt>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = bld<0>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u);
+        }
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void buf_store_strided(const cf (&v)[E], __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                  uint32_t stride, float scale, bool nt)
+{
+    if (nt)
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            bst<kAuxNt>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u, cf{v[m].x * scale, v[m].y * scale});
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            bst<0>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u, cf{v[m].x * scale, v[m].y * scale});
         }
     }
 }
@@ -145,7 +220,8 @@ __global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
     cf v[E];
     if (active)
     {
-        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + i, C::T, (a.hints & kHintNtLoad) != 0);
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + i, static_cast<uint32_t>(C::T),
+                        (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -159,7 +235,7 @@ __global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_rows(RowArgs a)
     fft_core<N, E, INV, 1>(v, i, view, reinterpret_cast<const cf*>(a.tw));
     if (active)
     {
-        store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + i, C::T, a.scale,
+        store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + i, static_cast<uint32_t>(C::T), a.scale,
                          (a.hints & kHintNtStore) != 0);
     }
 }
@@ -190,10 +266,12 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     const uint64_t col = (lb - t * groups) * CC + c;
     const bool active = t < a.count && col < a.ncols;
     cf v[E];
+    const uint32_t ncols = static_cast<uint32_t>(a.ncols); // n1 * ncols points per matrix fit 32 bits
+    const uint32_t off = static_cast<uint32_t>(col) + static_cast<uint32_t>(i) * ncols;
     if (active)
     {
-        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + col + static_cast<uint64_t>(i) * a.ncols,
-                        C::T * a.ncols, (a.hints & kHintNtLoad) != 0);
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + off, C::T * ncols,
+                        (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -209,26 +287,36 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     {
         return;
     }
-    if (a.thi != nullptr)
+    if (a.tfull != nullptr)
+    {
+        // W_N^{col k1} from the full table at [k1 * ncols + col]: one coalesced load per value
+        const cf* tf = reinterpret_cast<const cf*>(a.tfull) + off;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = ctw<INV>(v[m], tf[static_cast<uint32_t>(m) * (C::T * ncols)]);
+        }
+    }
+    else if (a.thi != nullptr)
     {
         // W_N^{col (i + T m)} = W_N^{col i} * W_N^{col T m}: one factored lookup per thread, then one coalesced
         // table load per value (lanes of adjacent columns read adjacent entries)
         const cf* thi = reinterpret_cast<const cf*>(a.thi);
         const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
         const cf* tstep = reinterpret_cast<const cf*>(a.tstep) + col;
-        const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
-        const uint64_t e0 = col * static_cast<uint64_t>(i);
+        const uint32_t lmask = (1u << a.hshift) - 1;
+        const uint32_t e0 = static_cast<uint32_t>(col) * static_cast<uint32_t>(i);
         const cf w0 = cmul(thi[e0 >> a.hshift], tlo[e0 & lmask]);
         v[0] = ctw<INV>(v[0], w0);
 #pragma unroll
         for (int m = 1; m < E; ++m)
         {
-            const cf w = cmul(w0, tstep[static_cast<uint64_t>(m) * a.ncols]);
+            const cf w = cmul(w0, tstep[static_cast<uint32_t>(m) * ncols]);
             v[m] = ctw<INV>(v[m], w);
         }
     }
-    store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + col + static_cast<uint64_t>(i) * a.ncols,
-                     C::T * a.ncols, 1.0f, (a.hints & kHintNtStore) != 0);
+    store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + off, C::T * ncols, 1.0f,
+                     (a.hints & kHintNtStore) != 0);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -262,8 +350,8 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     cf v[E];
     if (active)
     {
-        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2 + i, C::T,
-                        (a.hints & kHintNtLoad) != 0);
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2 + i,
+                        static_cast<uint32_t>(C::T), (a.hints & kHintNtLoad) != 0);
     }
     else
     {
@@ -303,8 +391,155 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
         const int k2 = j2 + C::T * m;
         w[m] = cf{tre[k2 * C::TP + r2], tim[k2 * C::TP + r2]};
     }
-    store_strided<E>(w, reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2 + static_cast<uint64_t>(j2) * a.n1,
-                     C::T * a.n1, a.scale, (a.hints & kHintNtStore) != 0);
+    const uint32_t n1 = static_cast<uint32_t>(a.n1);
+    store_strided<E>(w, reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2 + static_cast<uint32_t>(j2) * n1,
+                     C::T * n1, a.scale, (a.hints & kHintNtStore) != 0);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_fused: both four-step passes of one transform in one workgroup
+// ---------------------------------------------------------------------------------------------------------------
+// For n = N1 * N2 up to 2^17 the four-step intermediate of one transform (n * 8 bytes) is small enough to keep per
+// workgroup. A persistent grid of G workgroups, each owning one n-point slot of the workspace, runs transforms
+// t = block, block + G, ...:
+//   phase A  N2/CC tiles of CC adjacent columns: column FFT of N1 points, twiddle W_n^{col k1} (full table), store
+//            into the slot at [k1 * N2 + col];
+//   phase B  after a barrier, N1/RR tiles of RR rows of the slot: row FFT of N2 points, transposed store
+//            out[k1 + N1 k2] through LDS in runs of RR * 8 bytes.
+// The slot is written and read back by the same CU within one transform (G slots = 128-256 MiB in all), so it is
+// served by the L2 and the Infinity Cache and HBM sees one read and one write per point, where the two-kernel
+// four-step reads and writes every point twice. Each column tile's loads are issued before the previous tile is
+// transformed (and the next transform's first tile during the last column tile), so the HBM reads of the input
+// overlap the LDS stages of the workgroup's own compute.
+template <int N1, int E1, int N2, int E2, int TH> struct FusedCfg
+{
+    static constexpr int T1 = N1 / E1;
+    static constexpr int T2 = N2 / E2;
+    static constexpr int CC = TH / T1; // columns per phase-A tile
+    static constexpr int RR = TH / T2; // rows per phase-B tile
+    using A = ColCfg<N1, E1, CC>;
+    using B = RowTrCfg<N2, E2, RR>;
+    static_assert(A::threads == TH && B::threads == TH, "both phases use every thread");
+    static_assert(N2 % CC == 0 && N1 % RR == 0, "whole tiles");
+    static constexpr int data_bytes = A::lds_bytes > B::lds_bytes ? A::lds_bytes : B::lds_bytes;
+    // the stage twiddles of both lengths follow the data in LDS: global loads complete in order, so a table read
+    // from memory would make every stage wait for the prefetched tile as well
+    static constexpr int tw1_entries = static_cast<int>(Geometry<N1, E1>::twiddle_entries());
+    static constexpr int tw2_entries = static_cast<int>(Geometry<N2, E2>::twiddle_entries());
+    static constexpr int lds_bytes = data_bytes + (tw1_entries + tw2_entries) * 8;
+};
+
+template <int N1, int E1, int N2, int E2, int TH, bool INV>
+__global__ __launch_bounds__(TH) void k_fused(FusedArgs a)
+{
+    using F = FusedCfg<N1, E1, N2, E2, TH>;
+    constexpr int CC = F::CC;
+    constexpr int RR = F::RR;
+    constexpr int T1 = F::T1;
+    constexpr int T2 = F::T2;
+    constexpr uint32_t kN = static_cast<uint32_t>(N1) * N2;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int c = tid % CC; // phase A: column within the tile, slots i1 + T1 m
+    const int i1 = tid / CC;
+    const int r = tid / T2; // phase B: row within the tile, slots i2 + T2 m
+    const int i2 = tid - r * T2;
+    const int r2 = tid % RR; // phase B store: row r2, elements j2 + T2 m
+    const int j2 = tid / RR;
+    const LdsView<CC> va{lds + c, F::A::IM};
+    const LdsView<1> vb{lds + r * F::B::RS, F::B::LF};
+    float* tre = lds;
+    float* tim = lds + N2 * F::B::TP;
+    const cf* in = reinterpret_cast<const cf*>(a.in);
+    cf* out = reinterpret_cast<cf*>(a.out);
+    const __amdgpu_buffer_rsrc_t slot = make_rsrc(reinterpret_cast<cf*>(a.ws) + static_cast<uint64_t>(blockIdx.x) * kN);
+    const __amdgpu_buffer_rsrc_t tfull = make_rsrc(a.tfull);
+    cf* tw1 = reinterpret_cast<cf*>(lds + F::data_bytes / 4);
+    cf* tw2 = tw1 + F::tw1_entries;
+    for (int j = tid; j < F::tw1_entries; j += TH)
+    {
+        tw1[j] = reinterpret_cast<const cf*>(a.tw1)[j];
+    }
+    for (int j = tid; j < F::tw2_entries; j += TH)
+    {
+        tw2[j] = reinterpret_cast<const cf*>(a.tw2)[j];
+    }
+    __syncthreads();
+    const bool ntl = (a.hints & kHintNtLoad) != 0;
+    const bool nts = (a.hints & kHintNtStore) != 0;
+    const uint32_t a_off = static_cast<uint32_t>(c) + static_cast<uint32_t>(i1) * N2; // + tile * CC
+    constexpr uint32_t a_stride = static_cast<uint32_t>(T1) * N2;
+    const uint32_t b_off = static_cast<uint32_t>(r) * N2 + i2;                 // + tile * RR * N2
+    const uint32_t s_off = static_cast<uint32_t>(r2) + static_cast<uint32_t>(j2) * N1; // + tile * RR
+    constexpr uint32_t s_stride = static_cast<uint32_t>(T2) * N1;
+
+    uint64_t t = blockIdx.x;
+    cf nx[E1];
+    if (t < a.count)
+    {
+        buf_load_strided<E1>(nx, make_rsrc(in + t * a.idist), a_off, a_stride, ntl);
+    }
+#pragma unroll 1
+    for (; t < a.count; t += gridDim.x)
+    {
+        const __amdgpu_buffer_rsrc_t src = make_rsrc(in + t * a.idist);
+        const __amdgpu_buffer_rsrc_t dst = make_rsrc(out + t * a.odist);
+#pragma unroll 1
+        for (int ct = 0; ct < N2 / CC; ++ct)
+        {
+            cf v[E1];
+#pragma unroll
+            for (int m = 0; m < E1; ++m)
+            {
+                v[m] = nx[m];
+            }
+            if (ct + 1 < N2 / CC)
+            {
+                buf_load_strided<E1>(nx, src, a_off + (ct + 1) * CC, a_stride, ntl);
+            }
+            else if (t + gridDim.x < a.count)
+            {
+                buf_load_strided<E1>(nx, make_rsrc(in + (t + gridDim.x) * a.idist), a_off, a_stride, ntl);
+            }
+            fft_core<N1, E1, INV, CC>(v, i1, va, tw1);
+            const uint32_t off = a_off + static_cast<uint32_t>(ct) * CC;
+#pragma unroll
+            for (int m = 0; m < E1; ++m)
+            {
+                v[m] = ctw<INV>(v[m], bld<0>(tfull, off * 8u, static_cast<uint32_t>(m) * a_stride * 8u));
+            }
+            buf_store_strided<E1>(v, slot, off, a_stride, 1.0f, false);
+        }
+        __syncthreads(); // the slot is complete (and phase A's last LDS reads are done)
+#pragma unroll 1
+        for (int rt = 0; rt < N1 / RR; ++rt)
+        {
+            cf v[E2];
+            buf_load_strided<E2>(v, slot, b_off + static_cast<uint32_t>(rt * RR) * N2, static_cast<uint32_t>(T2),
+                                 false);
+            fft_core<N2, E2, INV, 1>(v, i2, vb, tw2);
+            if constexpr (N2 == E2)
+            {
+                __syncthreads();
+            }
+#pragma unroll
+            for (int m = 0; m < E2; ++m)
+            {
+                const int k2 = i2 + T2 * m;
+                tre[k2 * F::B::TP + r] = v[m].x;
+                tim[k2 * F::B::TP + r] = v[m].y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E2; ++m)
+            {
+                const int k2 = j2 + T2 * m;
+                v[m] = cf{tre[k2 * F::B::TP + r2], tim[k2 * F::B::TP + r2]};
+            }
+            buf_store_strided<E2>(v, dst, s_off + static_cast<uint32_t>(rt * RR), s_stride, a.scale, nts);
+            __syncthreads(); // LDS (and, after the last tile, the slot) free for the next tile or transform
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -330,6 +565,18 @@ __global__ void k_roots_lin(cf* out, uint64_t count, uint64_t mult, uint64_t d)
     if (j < count)
     {
         out[j] = root_fp64((j * mult) % d, d);
+    }
+}
+
+// Full four-step twiddle table: out[k1 * ncols + c] = W_n^{c k1} for k1 < rows.
+__global__ void k_roots_full(cf* out, uint64_t rows, uint64_t ncols, uint64_t n)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx < rows * ncols)
+    {
+        const uint64_t k1 = idx / ncols;
+        const uint64_t c = idx - k1 * ncols;
+        out[idx] = root_fp64(c * k1, n);
     }
 }
 
@@ -434,6 +681,11 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
     X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16)              \
         X(512, 16, 8) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
 
+// (N1, E1, N2, E2, threads) of the fused four-step
+#define CRDFFT_FUSED_LIST(X)                                                                                           \
+    X(128, 16, 256, 16, 1024) X(128, 16, 256, 16, 512) X(256, 16, 256, 16, 1024) X(256, 16, 256, 16, 512)            \
+        X(256, 16, 512, 16, 1024)
+
 constexpr uint64_t kMaxItems = 0xFFFFFFFFull; // work-items per launch
 
 template <typename K> hipError_t set_lds(K kernel, unsigned bytes)
@@ -474,6 +726,21 @@ bool col_kernel_shape(uint64_t n1, int e, int c, LaunchShape* out)
         return true;                                                                                                   \
     }
     CRDFFT_COL_LIST(X)
+#undef X
+    return false;
+}
+
+bool fused_kernel_shape(uint64_t n1, int e1, uint64_t n2, int e2, int threads, LaunchShape* out)
+{
+#define X(N1, E1, N2, E2, TH)                                                                                          \
+    if (n1 == N1 && e1 == E1 && n2 == N2 && e2 == E2 && threads == TH)                                                 \
+    {                                                                                                                  \
+        out->threads = TH;                                                                                             \
+        out->per_group = 1;                                                                                            \
+        out->lds_bytes = FusedCfg<N1, E1, N2, E2, TH>::lds_bytes;                                                      \
+        return true;                                                                                                   \
+    }
+    CRDFFT_FUSED_LIST(X)
 #undef X
     return false;
 }
@@ -528,6 +795,8 @@ hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a
     if (n1 == N && e == E && c == CC)                                                                                  \
     {                                                                                                                  \
         using C = ColCfg<N, E, CC>;                                                                                    \
+        if (static_cast<uint64_t>(N) * a.ncols >= (uint64_t{1} << 32))                                                 \
+            return hipErrorInvalidValue; /* the kernel's in-matrix offsets are 32-bit */                               \
         auto kern = inverse ? k_cols<N, E, CC, true> : k_cols<N, E, CC, false>;                                        \
         hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
         if (err != hipSuccess)                                                                                         \
@@ -582,6 +851,47 @@ hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs
     return hipErrorInvalidValue;
 }
 
+hipError_t fused_blocks_per_cu(uint64_t n1, int e1, uint64_t n2, int e2, int threads, int* blocks)
+{
+#define X(N1, E1, N2, E2, TH)                                                                                          \
+    if (n1 == N1 && e1 == E1 && n2 == N2 && e2 == E2 && threads == TH)                                                 \
+    {                                                                                                                  \
+        using F = FusedCfg<N1, E1, N2, E2, TH>;                                                                        \
+        auto kern = k_fused<N1, E1, N2, E2, TH, false>;                                                                \
+        hipError_t err = set_lds(kern, F::lds_bytes);                                                                  \
+        if (err == hipSuccess)                                                                                         \
+            err = set_lds(k_fused<N1, E1, N2, E2, TH, true>, F::lds_bytes);                                            \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, kern, TH, F::lds_bytes);                           \
+    }
+    CRDFFT_FUSED_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, bool inverse, const FusedArgs& a,
+                        uint64_t grid, hipStream_t s)
+{
+#define X(N1, E1, N2, E2, TH)                                                                                          \
+    if (n1 == N1 && e1 == E1 && n2 == N2 && e2 == E2 && threads == TH)                                                 \
+    {                                                                                                                  \
+        using F = FusedCfg<N1, E1, N2, E2, TH>;                                                                        \
+        if (grid == 0 || grid > a.slots)                                                                               \
+            return hipErrorInvalidValue; /* one workspace slot per workgroup */                                        \
+        auto kern = inverse ? k_fused<N1, E1, N2, E2, TH, true> : k_fused<N1, E1, N2, E2, TH, false>;                  \
+        hipError_t err = set_lds(kern, F::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(std::min<uint64_t>(grid, a.count))), dim3(TH),          \
+                           F::lds_bytes, s, a);                                                                        \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_FUSED_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 uint64_t stage_twiddle_entries(uint64_t n, int e)
 {
     if (n <= 1 || e <= 1 || static_cast<uint64_t>(e) >= n)
@@ -628,6 +938,18 @@ hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, h
     }
     const unsigned blocks = static_cast<unsigned>((count + 255) / 256);
     hipLaunchKernelGGL(k_roots_lin, dim3(blocks), dim3(256), 0, s, reinterpret_cast<cf*>(out), count, mult, d);
+    return hipGetLastError();
+}
+
+hipError_t build_full_twiddle(cfloat* out, uint64_t rows, uint64_t ncols, uint64_t n, hipStream_t s)
+{
+    const uint64_t count = rows * ncols;
+    if (count == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_roots_full, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<cf*>(out), rows, ncols, n);
     return hipGetLastError();
 }
 

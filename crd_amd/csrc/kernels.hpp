@@ -44,6 +44,7 @@ struct ColArgs
     const cfloat* thi;   // W_N^{j << hshift}
     const cfloat* tlo;   // W_N^{j}, j < 2^hshift
     const cfloat* tstep; // W_N^{n2 * (n1/E) * m} at [m * ncols + n2], m < E (the per-slot step of the twiddle)
+    const cfloat* tfull; // when set: W_N^{n2 k1} at [k1 * ncols + n2] (used instead of thi/tlo/tstep)
     uint32_t hshift;
     uint64_t ncols;
     uint64_t count;
@@ -67,6 +68,24 @@ struct RowTrArgs
     uint32_t hints;
 };
 
+// Both four-step passes of n = n1 * n2 in one persistent launch (k_fused): workgroup b runs transforms b, b + grid,
+// ... through its own n-point slot of `ws` (slots >= grid slots of n1 * n2 points).
+struct FusedArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    cfloat* ws;
+    const cfloat* tw1;   // stage twiddles of n1
+    const cfloat* tw2;   // stage twiddles of n2
+    const cfloat* tfull; // W_n^{c k1} at [k1 * n2 + c]
+    uint64_t slots;
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+    float scale;
+    uint32_t hints;
+};
+
 struct LaunchShape
 {
     unsigned threads = 0;     // workgroup size
@@ -79,7 +98,13 @@ bool row_kernel_shape(uint64_t n, int e, LaunchShape* out);
 bool col_kernel_shape(uint64_t n1, int e, int c, LaunchShape* out);
 bool rowtr_kernel_shape(uint64_t n2, int e, int r, LaunchShape* out);
 
+bool fused_kernel_shape(uint64_t n1, int e1, uint64_t n2, int e2, int threads, LaunchShape* out);
+// Resident workgroups per CU of the fused kernel (sets its LDS attribute first).
+hipError_t fused_blocks_per_cu(uint64_t n1, int e1, uint64_t n2, int e2, int threads, int* blocks);
+
 hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStream_t s);
+hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, bool inverse, const FusedArgs& a,
+                        uint64_t grid, hipStream_t s);
 hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
 hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs& a, hipStream_t s);
 
@@ -89,6 +114,8 @@ hipError_t build_stage_twiddles(cfloat* table, uint64_t n, int e, hipStream_t s)
 // out[j] = W_d^{j * mult} for j < count, computed in FP64 on the device and narrowed.
 hipError_t build_roots(cfloat* out, uint64_t count, uint64_t mult, uint64_t d, hipStream_t s);
 
+// The full four-step twiddle table out[k1 * ncols + c] = W_n^{c k1}, k1 < rows.
+hipError_t build_full_twiddle(cfloat* out, uint64_t rows, uint64_t ncols, uint64_t n, hipStream_t s);
 // The column pass's step table: out[m * ncols + n2] = W_n^{n2 * step * m}, m < e (step = n1 / e).
 hipError_t build_step_table(cfloat* out, uint64_t ncols, int e, uint64_t step, uint64_t n, hipStream_t s);
 

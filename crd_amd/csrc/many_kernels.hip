@@ -38,7 +38,7 @@ struct TileCtx
 };
 
 template <int R, bool INV>
-__device__ __forceinline__ void many_stage(const TileCtx& c, uint32_t p, bool first, bool last, int tid)
+__device__ __noinline__ void many_stage(const TileCtx& c, uint32_t p, bool first, bool last, int tid)
 {
     constexpr int QMAX = (kTilePoints / kThreads + R - 1) / R;
     const uint32_t nr = c.n / R;

@@ -314,7 +314,7 @@ crdfft_status crdfft_nd_execute(crdfft_nd_plan p, void* data, int32_t direction,
             }
             case kColumns:
             {
-                crdfft::ColArgs ca{x, x, static_cast<const crdfft::cfloat*>(a.tw), nullptr, nullptr, nullptr, 0,
+                crdfft::ColArgs ca{x, x, static_cast<const crdfft::cfloat*>(a.tw), nullptr, nullptr, nullptr, nullptr, 0,
                                    a.inner, blocks, a.d * a.inner, a.d * a.inner, 0};
                 ND_TRY(crdfft::launch_cols(a.d, a.e, a.c, inverse, ca, s));
                 break;

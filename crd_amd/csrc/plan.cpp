@@ -92,6 +92,7 @@ struct crdfft_plan_s
     DeviceBuffer tw1, tw2;  // stage tables of the column (n1) and row (n2) kernels
     DeviceBuffer thi, tlo;  // factored four-step twiddle
     DeviceBuffer tstep;     // column pass step table (e1 * n2 entries)
+    DeviceBuffer tfull;     // column pass full twiddle table (n entries, small n only)
     uint32_t hshift = 0;
     DeviceBuffer wreal;     // W_n^k, k <= n/2, for R2C/C2R
     DeviceBuffer wref;      // W_cn^k, k < cn/2, for the radix-2 reference (built on first use)
@@ -211,6 +212,35 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
         CRDFFT_TRY(p->tw_single.alloc(entries * sizeof(cfloat)));
         CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw_single.as<cfloat>(), cn, sh.e, s));
     }
+    else if (sh.algorithm == CRDFFT_ALGO_FUSED)
+    {
+        const uint64_t e1n = crdfft::stage_twiddle_entries(sh.n1, sh.e1);
+        const uint64_t e2n = crdfft::stage_twiddle_entries(sh.n2, sh.e2);
+        CRDFFT_TRY(p->tw1.alloc(e1n * sizeof(cfloat)));
+        CRDFFT_TRY(p->tw2.alloc(e2n * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw1.as<cfloat>(), sh.n1, sh.e1, s));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw2.as<cfloat>(), sh.n2, sh.e2, s));
+        CRDFFT_TRY(p->tfull.alloc(cn * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_full_twiddle(p->tfull.as<cfloat>(), sh.n1, sh.n2, cn, s));
+        // one workspace slot per workgroup that can be resident on this device
+        int per_cu = 0;
+        hipDeviceProp_t d{};
+        CRDFFT_TRY(crdfft::fused_blocks_per_cu(sh.n1, sh.e1, sh.n2, sh.e2, sh.threads, &per_cu));
+        CRDFFT_TRY(hipGetDeviceProperties(&d, p->device));
+        if (per_cu <= 0)
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+        // (the planner's count assumed 256 CUs and may also carry a tuned cap)
+        p->shape.chunk = std::min<uint64_t>(static_cast<uint64_t>(d.multiProcessorCount) * per_cu,
+                                            std::max<uint64_t>(sh.batch, 1));
+        if (sh.chunk != 0 && sh.chunk < p->shape.chunk)
+        {
+            p->shape.chunk = sh.chunk;
+        }
+        p->shape.workspace_bytes = p->shape.chunk * cn * sizeof(cfloat);
+        CRDFFT_TRY(p->workspace.alloc(p->shape.workspace_bytes));
+    }
     else if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP)
     {
         const uint64_t e1n = crdfft::stage_twiddle_entries(sh.n1, sh.e1);
@@ -229,6 +259,11 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
         CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
         CRDFFT_TRY(p->tstep.alloc(static_cast<uint64_t>(sh.e1) * sh.n2 * sizeof(cfloat)));
         CRDFFT_TRY(crdfft::build_step_table(p->tstep.as<cfloat>(), sh.n2, sh.e1, sh.n1 / sh.e1, cn, s));
+        if (cn <= crdfft::kFullTwiddleMax)
+        {
+            CRDFFT_TRY(p->tfull.alloc(cn * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_full_twiddle(p->tfull.as<cfloat>(), sh.n1, sh.n2, cn, s));
+        }
         CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
     }
     if (sh.type != CRDFFT_C2C)
@@ -275,6 +310,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                    p->thi.as<cfloat>(),
                                    p->tlo.as<cfloat>(),
                                    p->tstep.as<cfloat>(),
+                                   p->tfull.as<cfloat>(),
                                    p->hshift,
                                    sh.n2,
                                    cnt,
@@ -290,6 +326,25 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                 CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
                 CRDFFT_TRY(prof.after(1, cn * cnt));
             }
+            return CRDFFT_OK;
+        }
+        case CRDFFT_ALGO_FUSED:
+        {
+            crdfft::FusedArgs fa{in,
+                                 out,
+                                 p->workspace.as<cfloat>(),
+                                 p->tw1.as<cfloat>(),
+                                 p->tw2.as<cfloat>(),
+                                 p->tfull.as<cfloat>(),
+                                 sh.chunk,
+                                 count,
+                                 idist,
+                                 odist,
+                                 scale,
+                                 sh.hints};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_fused(sh.n1, sh.e1, sh.n2, sh.e2, sh.threads, inverse, fa, sh.chunk, s));
+            CRDFFT_TRY(prof.after(0, cn * count));
             return CRDFFT_OK;
         }
         default:
@@ -658,8 +713,8 @@ crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, 
         }
         tw = p->tw_em.as<cfloat>();
     }
-    crdfft::ColArgs a{static_cast<const cfloat*>(data), static_cast<cfloat*>(data), tw, nullptr, nullptr, nullptr, 0, batch,
-                      1, 0, 0, 0};
+    crdfft::ColArgs a{static_cast<const cfloat*>(data), static_cast<cfloat*>(data), tw, nullptr, nullptr, nullptr, nullptr,
+                      0, batch, 1, 0, 0, 0};
     CRDFFT_TRY(crdfft::launch_cols(sh.complex_n, e, c, inverse, a, s));
     return CRDFFT_OK;
 }
@@ -700,7 +755,7 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
     }
     crdfft::shape_to_info(p->shape, info);
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
-                        p->tstep.bytes + p->wreal.bytes + p->split_fwd.bytes + p->split_inv.bytes + p->tw256.bytes;
+                        p->tstep.bytes + p->tfull.bytes + p->wreal.bytes + p->split_fwd.bytes + p->split_inv.bytes + p->tw256.bytes;
     return CRDFFT_OK;
 }
 

@@ -38,6 +38,20 @@ bool have_rowtr_kernel(uint64_t n2, int e, int r)
     return rowtr_kernel_shape(n2, e, r, &s);
 }
 
+bool have_fused_kernel(uint64_t n1, int e1, uint64_t n2, int e2, int threads, unsigned* lds_bytes)
+{
+    LaunchShape s;
+    if (!fused_kernel_shape(n1, e1, n2, e2, threads, &s))
+    {
+        return false;
+    }
+    if (lds_bytes != nullptr)
+    {
+        *lds_bytes = s.lds_bytes;
+    }
+    return true;
+}
+
 namespace
 {
 
@@ -163,7 +177,8 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
         {
             s.e = tuning->e;
         }
-        if (!have_row_kernel(cn, s.e))
+        if (!have_row_kernel(cn, s.e) ||
+            (tuning != nullptr && tuning->algorithm != 0 && tuning->algorithm != CRDFFT_ALGO_SINGLE))
         {
             return CRDFFT_UNSUPPORTED;
         }
@@ -198,12 +213,52 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
             if (tuning->tile2 != 0)
                 s.tile2 = tuning->tile2;
         }
+        const uint64_t per = cn * 8;
+        if (tuning != nullptr && tuning->algorithm == CRDFFT_ALGO_FUSED)
+        {
+            // both passes in one workgroup: tile1 columns per phase-A tile set the workgroup size, the rows per
+            // phase-B tile follow from it
+            if (cn > kFusedMax)
+            {
+                return CRDFFT_UNSUPPORTED;
+            }
+            const int t1 = static_cast<int>(s.n1) / s.e1;
+            const int t2 = static_cast<int>(s.n2) / s.e2;
+            const int threads = (tuning->tile1 != 0) ? t1 * tuning->tile1 : kFusedThreads;
+            unsigned lds = 0;
+            if (threads % t2 != 0 || !have_fused_kernel(s.n1, s.e1, s.n2, s.e2, threads, &lds))
+            {
+                return CRDFFT_UNSUPPORTED;
+            }
+            s.algorithm = CRDFFT_ALGO_FUSED;
+            s.threads = threads;
+            s.tile1 = threads / t1;
+            s.tile2 = threads / t2;
+            // one slot per resident workgroup (the plan recounts them on its device)
+            const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(2048 / threads, (160u * 1024u) / lds));
+            s.chunk = std::min<uint64_t>(kDescribeCUs * per_cu, std::max<uint64_t>(batch, 1));
+            if (tuning->chunk != 0)
+            {
+                s.chunk = std::min(s.chunk, tuning->chunk); // fewer slots than resident workgroups
+            }
+            s.workspace_bytes = s.chunk * per;
+            s.hints = kDefaultFourStepHints;
+            if (tuning->hints != 0)
+            {
+                s.hints = tuning->hints & 3u;
+            }
+            *out = s;
+            return CRDFFT_OK;
+        }
+        if (tuning != nullptr && tuning->algorithm != 0 && tuning->algorithm != CRDFFT_ALGO_FOUR_STEP)
+        {
+            return CRDFFT_INVALID_ARGUMENT;
+        }
         if (!have_col_kernel(s.n1, s.e1, s.tile1) || !have_rowtr_kernel(s.n2, s.e2, s.tile2) ||
             (s.n1 % static_cast<uint64_t>(s.tile2)) != 0)
         {
             return CRDFFT_UNSUPPORTED;
         }
-        const uint64_t per = cn * 8;
         s.chunk = std::max<uint64_t>(1, kChunkBytes / per);
         if (tuning != nullptr && tuning->chunk != 0)
         {

@@ -29,6 +29,7 @@ struct PlanShape
     uint64_t chunk = 0;
     uint64_t workspace_bytes = 0;
     uint32_t hints = 0; // kHintNtLoad | kHintNtStore
+    int32_t threads = 0; // fused four-step: workgroup size (tile1 columns, tile2 rows per tile)
 };
 
 // The largest complex length one LDS-resident kernel runs.
@@ -41,6 +42,16 @@ constexpr uint64_t kFourStepMax = uint64_t{1} << 24;
 constexpr uint64_t kChunkBytes = uint64_t{256} << 20;
 // Cache hints of the four-step: bit 0 streams the column pass's input past the caches, bit 1 the row pass's output.
 constexpr uint32_t kDefaultFourStepHints = 3;
+// The largest complex length of the fused four-step (one transform's intermediate per workgroup slot).
+constexpr uint64_t kFusedMax = uint64_t{1} << 17;
+// Workgroup size of the fused four-step unless tuned: 512 threads (two workgroups per CU hide each other's memory
+// waits).
+constexpr int kFusedThreads = 512;
+// CUs assumed by crdfft_plan_describe for the fused slot count (plan creation asks the device).
+constexpr uint64_t kDescribeCUs = 256;
+// Up to this length the column pass reads its twiddle W_N^{n2 k1} from one full table (n entries, at most 2 MiB,
+// L2-resident): one load and one complex multiply per value instead of two multiplies.
+constexpr uint64_t kFullTwiddleMax = uint64_t{1} << 18;
 
 bool is_pow2(uint64_t n);
 uint32_t log2_exact(uint64_t n);
@@ -56,5 +67,7 @@ bool element_major_shape(uint64_t n, uint64_t batch, int* e, int* c);
 bool have_row_kernel(uint64_t n, int e);
 bool have_col_kernel(uint64_t n1, int e, int c);
 bool have_rowtr_kernel(uint64_t n2, int e, int r);
+// Fused four-step of n1 x n2 at threads per workgroup; lds_bytes receives its LDS per workgroup.
+bool have_fused_kernel(uint64_t n1, int e1, uint64_t n2, int e2, int threads, unsigned* lds_bytes = nullptr);
 
 } // namespace crdfft

@@ -66,6 +66,7 @@ crdfft_tuning same_kernels(const crdfft_plan_info& info)
     t.e2 = info.e2;
     t.tile2 = info.tile2;
     t.chunk = 1;
+    t.algorithm = info.algorithm;
     return t;
 }
 

@@ -22,7 +22,7 @@ namespace
 bool same_shape(const crdfft::PlanShape& a, const crdfft::PlanShape& b)
 {
     return a.algorithm == b.algorithm && a.n1 == b.n1 && a.e == b.e && a.e1 == b.e1 && a.tile1 == b.tile1 &&
-           a.e2 == b.e2 && a.tile2 == b.tile2 && a.chunk == b.chunk && a.hints == b.hints;
+           a.e2 == b.e2 && a.tile2 == b.tile2 && a.chunk == b.chunk && a.hints == b.hints && a.threads == b.threads;
 }
 
 crdfft_tuning tuning_of(const crdfft::PlanShape& s)
@@ -36,6 +36,7 @@ crdfft_tuning tuning_of(const crdfft::PlanShape& s)
     t.tile2 = s.tile2;
     t.chunk = s.chunk;
     t.hints = s.hints;
+    t.algorithm = s.algorithm;
     return t;
 }
 
@@ -79,9 +80,21 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
             }
         }
     }
-    else if (def.algorithm == CRDFFT_ALGO_FOUR_STEP)
+    else if (def.algorithm == CRDFFT_ALGO_FOUR_STEP || def.algorithm == CRDFFT_ALGO_FUSED)
     {
         const uint32_t lg = crdfft::log2_exact(cn);
+        // the fused four-step at its instantiated workgroup sizes (tile1 = columns per phase-A tile)
+        for (int tile1 : {0, 32, 64, 128})
+        {
+            for (uint32_t hints : {1u, 3u})
+            {
+                crdfft_tuning t{};
+                t.algorithm = CRDFFT_ALGO_FUSED;
+                t.tile1 = tile1;
+                t.hints = hints;
+                add(&t);
+            }
+        }
         for (int d = -1; d <= 2; ++d)
         {
             const int l1 = static_cast<int>(lg / 2) + d;
@@ -96,6 +109,7 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
                     for (int variant = 0; variant < 3; ++variant)
                     {
                         crdfft_tuning t{};
+                        t.algorithm = CRDFFT_ALGO_FOUR_STEP;
                         t.n1 = uint64_t{1} << l1;
                         t.chunk = std::max<uint64_t>(1, (mib << 20) / (cn * 8));
                         t.hints = hints;
@@ -232,6 +246,7 @@ crdfft_status crdfft_time_plan(uint64_t n, uint64_t batch, int32_t type, const c
     record->tuning.tile2 = info.tile2;
     record->tuning.chunk = info.chunk;
     record->tuning.hints = info.hints;
+    record->tuning.algorithm = info.algorithm;
     record->algorithm = info.algorithm;
     record->ms = ms[ms.size() / 2];
     return CRDFFT_OK;
@@ -334,7 +349,7 @@ crdfft_status crdfft_plan_features(const crdfft_plan_record* records, uint64_t c
         f[28] = r.ndim >= 2 ? log2f_safe(static_cast<double>(r.dims[0])) / 12.0f : 0.0f;
         f[29] = r.ndim >= 3 ? log2f_safe(static_cast<double>(r.dims[1])) / 12.0f : 0.0f;
         f[30] = r.algorithm == CRDFFT_ALGO_FOUR_STEP ? 1.0f : 0.0f;
-        f[31] = 1.0f; // bias-like constant input
+        f[31] = r.algorithm == CRDFFT_ALGO_FUSED ? 1.0f : 0.0f;
     }
     return CRDFFT_OK;
 }

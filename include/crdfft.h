@@ -81,7 +81,9 @@ typedef enum crdfft_algorithm
     CRDFFT_ALGO_SINGLE = 1,     /* one LDS-staged kernel per transform (n <= 16384) */
     CRDFFT_ALGO_FOUR_STEP = 2,  /* n = n1*n2: column pass with twiddle, row pass with transposed store */
     CRDFFT_ALGO_REFERENCE = 3,  /* the radix-2 reference only */
-    CRDFFT_ALGO_BLUESTEIN = 4   /* chirp-z over a power-of-two plan of m >= 2n-1 (any n) */
+    CRDFFT_ALGO_BLUESTEIN = 4,  /* chirp-z over a power-of-two plan of m >= 2n-1 (any n) */
+    CRDFFT_ALGO_FUSED = 5       /* n = n1*n2 <= 2^17: both four-step passes of a transform in one workgroup of a
+                                   persistent launch, the intermediate in a per-workgroup slot (chunk = slots) */
 } crdfft_algorithm;
 
 /* Plan flags */
@@ -117,6 +119,9 @@ typedef struct crdfft_tuning
     int32_t e, e1, tile1, e2, tile2;
     uint64_t chunk;       /* transforms per pass pair */
     uint32_t hints;       /* cache hints (crdfft_plan_info.hints); 0 keeps the planner's */
+    int32_t algorithm;    /* CRDFFT_ALGO_FOUR_STEP or CRDFFT_ALGO_FUSED for n > 16384; 0 keeps the planner's.
+                             FUSED: tile1 = columns per phase-A tile (threads = n1/e1 * tile1), tile2 derived,
+                             chunk caps the persistent workgroups (slots) */
 } crdfft_tuning;
 
 /* A plan of `batch` transforms of length n. idist/odist: distance between consecutive transforms in elements of
