@@ -291,7 +291,8 @@ def main():
         for _ in range(3):
             acc = p.execute_profiled(xin, xout, crd_amd.FORWARD, stream, acc)
         info = p.info()
-        names = ["single" if info["algorithm"] == 1 else "column pass", "row pass (transposed store)", "real", "copy"]
+        first_phase = {1: "single", 5: "fused four-step (both passes)"}.get(info["algorithm"], "column pass")
+        names = [first_phase, "row pass (transposed store)", "real", "copy"]
         kernels = []
         for ph in range(4):
             if acc.launches[ph] == 0:
@@ -353,10 +354,16 @@ def main():
 
     if not args.no_extras:
         extras = {}
-        # device copy ceiling (read + write bytes)
+        # device copy ceiling (read + write bytes): the library's streaming copy kernel (one 16-byte element per
+        # work-item) and, beside it, hipMemcpyAsync device to device
+        from crd_amd import _native as N
+
         xin, xout = bufs[0]
-        ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
+        ms = time_loop(lambda: N.check(N.lib().crdfft_stream_copy(xout.ptr, xin.ptr, xin.nbytes, stream.handle),
+                                       "crdfft_stream_copy"), stream, 5, 1, Event)
         extras["device_copy_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
+        ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
+        extras["hip_memcpy_d2d_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
         # R2C -> C2R round trip of both shapes
         rts = []
         for idx, (n, batch) in enumerate(SHAPES):

@@ -15,6 +15,7 @@
 //
 // Every kernel guards its own range (t < count, column < ncols) and runs its barriers unconditionally, so a
 // partial last workgroup is safe.
+#include "buffer.hpp"
 #include "fft_core.hpp"
 #include "kernels.hpp"
 
@@ -43,8 +44,6 @@ __device__ __forceinline__ uint64_t xcd_block()
 }
 
 constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
-typedef float f2v __attribute__((ext_vector_type(2)));
 
 template <bool NT> __device__ __forceinline__ cf ldg(const cf* p)
 {
@@ -115,36 +114,6 @@ __device__ __forceinline__ void store_strided(const cf (&v)[E], cf* dst, uint32_
             stg<false>(dst + static_cast<uint32_t>(m) * stride, cf{v[m].x * scale, v[m].y * scale});
         }
     }
-}
-
-// Raw buffer addressing: the base address sits in a 128-bit resource in scalar registers and every access takes a
-// 32-bit byte offset per lane plus a scalar offset, so the E accesses of a thread share one address VGPR and cost
-// no vector address arithmetic (the per-m stride goes into the scalar offset). Offsets stay below 4 GiB from the
-// base, which the launchers guarantee by making one resource per matrix.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
-}
-
-// The builtins' own 8-byte vector type (a generic vector, converted as a whole: its elements are not accessed).
-using b64_t = decltype(__builtin_amdgcn_raw_buffer_load_b64(make_rsrc(nullptr), 0, 0, 0));
-static_assert(sizeof(b64_t) == 8, "64-bit buffer access");
-constexpr int kAuxNt = 2; // cache policy of the buffer instructions: non-temporal
-
-template <int AUX> __device__ __forceinline__ cf bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    const b64_t u = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(voff), static_cast<int>(soff), AUX);
-    const f2v f = __builtin_bit_cast(f2v, u);
-    return cf{f.x, f.y};
-}
-
-template <int AUX> __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, cf v)
-{
-    f2v f;
-    f.x = v.x;
-    f.y = v.y;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64_t, f), r, static_cast<int>(voff),
-                                          static_cast<int>(soff), AUX);
 }
 
 // v[m] = element (off + m * stride) of the resource's array, for m < E (offsets in elements).
@@ -317,6 +286,112 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     }
     store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + off, C::T * ncols, 1.0f,
                      (a.hints & kHintNtStore) != 0);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_cols_pf: the column pass as a persistent launch that loads the next tile while it transforms this one
+// ---------------------------------------------------------------------------------------------------------------
+// For long columns (n1 >= 1024) one k_cols workgroup fills a CU's LDS, so nothing overlaps its load phase with its
+// compute phase and the CU alternates between waiting on HBM and running LDS stages. Here the grid is the resident
+// capacity; workgroup b runs tiles (matrix t, column group g) of one contiguous per-XCD range in steps of the
+// XCD's workgroup count, and issues the loads of its next tile before the FFT of the current one. The stage
+// twiddles sit in LDS (global loads complete in order: a table read from memory would wait for the prefetched tile).
+template <int N1, int E, int CC> struct ColPfCfg
+{
+    using C = ColCfg<N1, E, CC>;
+    static constexpr int tw_entries = static_cast<int>(Geometry<N1, E>::twiddle_entries());
+    static constexpr int lds_bytes = C::lds_bytes + tw_entries * 8;
+};
+
+// TWM: the four-step twiddle of the tile, 1 = full table (a.tfull), 2 = factored (a.thi, a.tlo, a.tstep)
+template <int N1, int E, int CC, int TWM, bool INV>
+__global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols_pf(ColArgs a)
+{
+    using C = ColCfg<N1, E, CC>;
+    using P = ColPfCfg<N1, E, CC>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int c = tid % CC;
+    const int i = tid / CC;
+    cf* tw = reinterpret_cast<cf*>(lds + C::lds_bytes / 4);
+    for (int j = tid; j < P::tw_entries; j += C::threads)
+    {
+        tw[j] = reinterpret_cast<const cf*>(a.tw)[j];
+    }
+    __syncthreads();
+    // tiles of this XCD: [lo, hi), taken in steps of the XCD's workgroup count (the launcher makes the grid a
+    // multiple of 8; workgroup b runs on XCD b % 8)
+    const uint32_t groups = static_cast<uint32_t>(a.ncols / CC);
+    const uint64_t tiles = a.count * groups;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t per_xcd = gridDim.x >> 3;
+    const uint64_t lo = tiles * xcd / 8;
+    const uint64_t hi = tiles * (xcd + 1) / 8;
+    const uint32_t ncols = static_cast<uint32_t>(a.ncols);
+    const uint32_t stride = static_cast<uint32_t>(C::T) * ncols;
+    const uint32_t ioff = static_cast<uint32_t>(c) + static_cast<uint32_t>(i) * ncols; // + g * CC
+    const bool ntl = (a.hints & kHintNtLoad) != 0;
+    const bool nts = (a.hints & kHintNtStore) != 0;
+    const cf* in = reinterpret_cast<const cf*>(a.in);
+    cf* out = reinterpret_cast<cf*>(a.out);
+    const LdsView<CC> view{lds + c, C::IM};
+
+    uint64_t tile = lo + (blockIdx.x >> 3);
+    cf nx[E];
+    if (tile < hi)
+    {
+        const uint64_t t = tile / groups;
+        const uint32_t g = static_cast<uint32_t>(tile - t * groups);
+        buf_load_strided<E>(nx, make_rsrc(in + t * a.idist), ioff + g * CC, stride, ntl);
+    }
+#pragma unroll 1
+    for (; tile < hi; tile += per_xcd)
+    {
+        const uint64_t t = tile / groups;
+        const uint32_t g = static_cast<uint32_t>(tile - t * groups);
+        cf v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = nx[m];
+        }
+        const uint64_t next = tile + per_xcd;
+        if (next < hi)
+        {
+            const uint64_t tn = next / groups;
+            const uint32_t gn = static_cast<uint32_t>(next - tn * groups);
+            buf_load_strided<E>(nx, make_rsrc(in + tn * a.idist), ioff + gn * CC, stride, ntl);
+        }
+        fft_core<N1, E, INV, CC>(v, i, view, tw);
+        const uint32_t col = g * CC + c;
+        const uint32_t off = ioff + g * CC;
+        if constexpr (TWM == 1)
+        {
+            const __amdgpu_buffer_rsrc_t tf = make_rsrc(a.tfull);
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                v[m] = ctw<INV>(v[m], bld<0>(tf, off * 8u, static_cast<uint32_t>(m) * stride * 8u));
+            }
+        }
+        else
+        {
+            const cf* thi = reinterpret_cast<const cf*>(a.thi);
+            const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+            const __amdgpu_buffer_rsrc_t ts = make_rsrc(a.tstep);
+            const uint32_t lmask = (1u << a.hshift) - 1;
+            const uint32_t e0 = col * static_cast<uint32_t>(i);
+            const cf w0 = cmul(thi[e0 >> a.hshift], tlo[e0 & lmask]);
+            v[0] = ctw<INV>(v[0], w0);
+#pragma unroll
+            for (int m = 1; m < E; ++m)
+            {
+                const cf w = cmul(w0, bld<0>(ts, col * 8u, static_cast<uint32_t>(m) * ncols * 8u));
+                v[m] = ctw<INV>(v[m], w);
+            }
+        }
+        buf_store_strided<E>(v, make_rsrc(out + t * a.odist), off, stride, 1.0f, nts);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -643,10 +718,14 @@ __global__ void k_ref_stage(cf* data, const cf* w, uint64_t n, uint64_t len, uin
     base[k + half] = csub(av, b);
 }
 
-__global__ void k_stream_copy(const float4* __restrict__ in, float4* __restrict__ out, uint64_t count)
+// One 16-byte element per work-item over a grid that covers the buffer: every wave's loads are issued at once and
+// the dispatcher keeps the CUs full (a grid-stride loop of one load per iteration leaves too few bytes in flight and
+// measures ~20% below this on MI355X).
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ in, float4* __restrict__ out,
+                                                     uint64_t count)
 {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < count)
     {
         out[i] = in[i];
     }
@@ -680,6 +759,10 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 #define CRDFFT_ROWTR_LIST(X)                                                                                           \
     X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16)              \
         X(512, 16, 8) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+
+// (N1, E, C) of the persistent prefetching column pass
+#define CRDFFT_COL_PF_LIST(X)                                                                                          \
+    X(256, 16, 16) X(512, 16, 16) X(1024, 16, 8) X(1024, 16, 16) X(1024, 32, 16) X(2048, 16, 8)
 
 // (N1, E1, N2, E2, threads) of the fused four-step
 #define CRDFFT_FUSED_LIST(X)                                                                                           \
@@ -816,6 +899,54 @@ hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a
         return hipGetLastError();                                                                                      \
     }
     CRDFFT_COL_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+bool have_cols_pf(uint64_t n1, int e, int c)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+        return true;
+    CRDFFT_COL_PF_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_cols_pf(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+    {                                                                                                                  \
+        using P = ColPfCfg<N, E, CC>;                                                                                  \
+        using C = ColCfg<N, E, CC>;                                                                                    \
+        /* whole column groups, byte offsets of one matrix below 4 GiB */                                              \
+        if (a.ncols % CC != 0 || static_cast<uint64_t>(N) * a.ncols * 8 >= (uint64_t{1} << 32) ||                      \
+            a.count * (a.ncols / CC) >= (uint64_t{1} << 40))                                                           \
+            return hipErrorInvalidValue;                                                                               \
+        if (a.tfull == nullptr && a.thi == nullptr)                                                                    \
+            return hipErrorInvalidValue; /* four-step columns only */                                                  \
+        auto kern = a.tfull != nullptr ? (inverse ? k_cols_pf<N, E, CC, 1, true> : k_cols_pf<N, E, CC, 1, false>)      \
+                                       : (inverse ? k_cols_pf<N, E, CC, 2, true> : k_cols_pf<N, E, CC, 2, false>);     \
+        hipError_t err = set_lds(kern, P::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        int dev = 0, cus = 0, per_cu = 0;                                                                              \
+        err = hipGetDevice(&dev);                                                                                      \
+        if (err == hipSuccess)                                                                                         \
+            err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);                             \
+        if (err == hipSuccess)                                                                                         \
+            err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, C::threads, P::lds_bytes);              \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        if (per_cu < 1)                                                                                                \
+            return hipErrorInvalidValue;                                                                               \
+        /* the tile walk needs a multiple of 8 workgroups (one range per XCD) */                                       \
+        const uint64_t grid = std::max<uint64_t>(8, (static_cast<uint64_t>(cus) * per_cu) & ~uint64_t{7});             \
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(C::threads), P::lds_bytes, s, a);            \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_COL_PF_LIST(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1000,14 +1131,14 @@ hipError_t launch_stream_copy(void* dst, const void* src, uint64_t bytes, hipStr
     {
         return hipErrorInvalidValue;
     }
-    const uint64_t count = bytes / 16;
-    const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>((count + 255) / 256, 256 * 32));
-    if (count == 0)
+    const uint64_t total = bytes / 16;
+    const uint64_t max_items = (kMaxItems / 256) * 256;
+    for (uint64_t done = 0; done < total; done += max_items)
     {
-        return hipSuccess;
+        const uint64_t count = std::min(total - done, max_items);
+        hipLaunchKernelGGL(k_stream_copy, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const float4*>(src) + done, static_cast<float4*>(dst) + done, count);
     }
-    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(src),
-                       static_cast<float4*>(dst), count);
     return hipGetLastError();
 }
 

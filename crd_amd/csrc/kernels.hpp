@@ -32,6 +32,9 @@ struct RowArgs
 // cache-resident intermediate).
 constexpr uint32_t kHintNtLoad = 1u;
 constexpr uint32_t kHintNtStore = 2u;
+// Plan knob beside the cache hints: run the four-step's column pass as a persistent launch that prefetches its next
+// tile (k_cols_pf), where that kernel is instantiated.
+constexpr uint32_t kHintPersistent = 8u;
 
 // Column transforms of length n1 over `ncols` adjacent columns of a row-major n1 x ncols matrix, for `count`
 // matrices at t*dist: four-step pass 1 (with the factored twiddle W_N^{n2 k1} when thi != nullptr) and the
@@ -106,6 +109,9 @@ hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStr
 hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, bool inverse, const FusedArgs& a,
                         uint64_t grid, hipStream_t s);
 hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
+// The persistent prefetching column pass (same arithmetic as launch_cols; ncols a multiple of c).
+bool have_cols_pf(uint64_t n1, int e, int c);
+hipError_t launch_cols_pf(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
 hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs& a, hipStream_t s);
 
 // Entries of the stage twiddle table of an n-point kernel at e values per thread, and the FP64 device build of it.

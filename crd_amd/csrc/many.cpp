@@ -71,6 +71,9 @@ bool smooth(uint64_t n, Exponents* e)
     return n == 1;
 }
 
+// Transforms of length n in one full tile.
+uint32_t max_group(uint64_t n) { return static_cast<uint32_t>(crdfft::kManyTilePoints / n); }
+
 // Candidates: the power-of-two part in radices of at most `cap` (16, 8, 4; one smaller radix for the remaining
 // bits), the odd part as 7s, 5s and 3s; powers of two first or odd radices first; full tiles or half tiles.
 std::vector<crdfft_radix_plan> candidates(uint64_t n)
@@ -81,7 +84,7 @@ std::vector<crdfft_radix_plan> candidates(uint64_t n)
     {
         return out;
     }
-    const uint32_t full = static_cast<uint32_t>(crdfft::kManyTilePoints / n);
+    const uint32_t full = max_group(n);
     for (uint32_t capbits : {4u, 3u, 2u})
     {
         std::vector<uint32_t> p2;
@@ -155,7 +158,7 @@ struct DevBuf
 struct crdfft_many_plan_s
 {
     crdfft_many_info info{};
-    DevBuf sizes, tiles, offsets, tw, wg_begin;
+    DevBuf sizes, tiles, tw, wg_begin;
     unsigned workgroups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -182,7 +185,7 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
     std::map<uint64_t, uint32_t> id_of;
     std::vector<crdfft::ManySize> sizes;
     std::vector<uint32_t> group_of;
-    std::vector<std::vector<uint64_t>> members;
+    std::vector<uint32_t> size_of(count);
     std::vector<uint64_t> offs(count);
     uint64_t run = 0;
     for (uint64_t t = 0; t < count; ++t)
@@ -212,7 +215,7 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
             }
             // a supplied plan must multiply out to n, use only known radices and fit a tile
             uint64_t prod = 1;
-            for (uint32_t s = 0; s < chosen.nstages; ++s)
+            for (uint32_t s = 0; s < chosen.nstages && s < static_cast<uint32_t>(crdfft::kManyMaxStages); ++s)
             {
                 const uint32_t r = chosen.radix[s];
                 if (r != 1 && r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16)
@@ -221,7 +224,7 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
                 }
                 prod *= r;
             }
-            const uint32_t full = static_cast<uint32_t>(crdfft::kManyTilePoints / n);
+            const uint32_t full = max_group(n);
             if (chosen.nstages == 0 || chosen.nstages > static_cast<uint32_t>(crdfft::kManyMaxStages) || prod != n ||
                 chosen.group > full)
             {
@@ -229,45 +232,54 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
             }
             crdfft::ManySize sz{};
             sz.n = static_cast<uint32_t>(n);
+            sz.n_magic = crdfft::many_magic(sz.n);
             sz.nstages = chosen.nstages;
-            std::copy(chosen.radix, chosen.radix + chosen.nstages, sz.radix);
+            uint32_t pp = 1;
+            for (uint32_t s = 0; s < chosen.nstages; ++s)
+            {
+                crdfft::ManyStage& st = sz.stage[s];
+                st.radix = chosen.radix[s];
+                st.nr = sz.n / st.radix;
+                st.nr_magic = crdfft::many_magic(st.nr);
+                st.p = pp;
+                st.p_magic = crdfft::many_magic(pp);
+                st.twstep = sz.n / (pp * st.radix);
+                pp *= st.radix;
+            }
             sz.tw_offset = tw_total;
-            tw_total += n;
+            tw_total += crdfft::kManyTwEntries;
             it = id_of.emplace(n, static_cast<uint32_t>(sizes.size())).first;
             sizes.push_back(sz);
             group_of.push_back(chosen.group == 0 ? full : chosen.group);
-            members.emplace_back();
         }
-        members[it->second].push_back(t);
+        size_of[t] = it->second;
     }
-    // tiles: runs of up to `group` transforms of one length
+    // tiles: runs of at most `group` transforms of one length that are contiguous in memory, in the given order
     struct HostTile
     {
         crdfft::ManyTile tile;
-        std::vector<uint64_t> offs;
         double cost;
     };
     std::vector<HostTile> tiles;
     uint64_t points = 0;
-    for (size_t id = 0; id < sizes.size(); ++id)
+    for (uint64_t t = 0; t < count;)
     {
+        const uint32_t id = size_of[t];
         const uint64_t n = sizes[id].n;
-        const std::vector<uint64_t>& mem = members[id];
-        for (size_t i = 0; i < mem.size(); i += group_of[id])
+        uint64_t cnt = 1;
+        while (t + cnt < count && cnt < group_of[id] && size_of[t + cnt] == id && offs[t + cnt] == offs[t] + cnt * n)
         {
-            HostTile h;
-            const size_t cnt = std::min<size_t>(group_of[id], mem.size() - i);
-            h.tile.size_id = static_cast<uint32_t>(id);
-            h.tile.count = static_cast<uint32_t>(cnt);
-            for (size_t j = 0; j < cnt; ++j)
-            {
-                h.offs.push_back(offs[mem[i + j]]);
-            }
-            // cost: memory (8 B in + out per point) plus a fixed tile latency, in point units
-            h.cost = static_cast<double>(cnt * n) * (1.0 + 0.05 * sizes[id].nstages) + 2048.0;
-            points += cnt * n;
-            tiles.push_back(std::move(h));
+            ++cnt;
         }
+        HostTile h;
+        h.tile.size_id = id;
+        h.tile.count = static_cast<uint32_t>(cnt);
+        h.tile.base = offs[t];
+        // cost: memory and stages per point plus a fixed tile latency, in point units
+        h.cost = static_cast<double>(cnt * n) * (1.0 + 0.05 * sizes[id].nstages) + 2048.0;
+        points += cnt * n;
+        tiles.push_back(h);
+        t += cnt;
     }
     int dev = 0, cus = 0;
     MANY_TRY(hipGetDevice(&dev));
@@ -295,33 +307,30 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
         heap.push(l);
     }
     std::vector<crdfft::ManyTile> flat_tiles;
-    std::vector<uint64_t> flat_offs;
     std::vector<uint32_t> begin(wgs + 1, 0);
     for (unsigned w = 0; w < wgs; ++w)
     {
         begin[w] = static_cast<uint32_t>(flat_tiles.size());
         for (size_t idx : per_wg[w])
         {
-            crdfft::ManyTile t = tiles[idx].tile;
-            t.first = flat_offs.size();
-            flat_offs.insert(flat_offs.end(), tiles[idx].offs.begin(), tiles[idx].offs.end());
-            flat_tiles.push_back(t);
+            flat_tiles.push_back(tiles[idx].tile);
         }
     }
     begin[wgs] = static_cast<uint32_t>(flat_tiles.size());
     MANY_TRY(p->sizes.alloc(sizes.size() * sizeof(crdfft::ManySize)));
     MANY_TRY(p->tiles.alloc(flat_tiles.size() * sizeof(crdfft::ManyTile)));
-    MANY_TRY(p->offsets.alloc(flat_offs.size() * sizeof(uint64_t)));
     MANY_TRY(p->tw.alloc(tw_total * sizeof(crdfft::cfloat)));
     MANY_TRY(p->wg_begin.alloc(begin.size() * sizeof(uint32_t)));
     MANY_TRY(hipMemcpy(p->sizes.p, sizes.data(), sizes.size() * sizeof(crdfft::ManySize), hipMemcpyHostToDevice));
     MANY_TRY(hipMemcpy(p->tiles.p, flat_tiles.data(), flat_tiles.size() * sizeof(crdfft::ManyTile),
                        hipMemcpyHostToDevice));
-    MANY_TRY(hipMemcpy(p->offsets.p, flat_offs.data(), flat_offs.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
     MANY_TRY(hipMemcpy(p->wg_begin.p, begin.data(), begin.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     for (const crdfft::ManySize& sz : sizes)
     {
-        MANY_TRY(crdfft::build_roots(static_cast<crdfft::cfloat*>(p->tw.p) + sz.tw_offset, sz.n, 1, sz.n, nullptr));
+        // lo[j] = W_n^j, hi[i] = W_n^{128 i}, both from FP64
+        crdfft::cfloat* t = static_cast<crdfft::cfloat*>(p->tw.p) + sz.tw_offset;
+        MANY_TRY(crdfft::build_roots(t, crdfft::kManyTwLo, 1, sz.n, nullptr));
+        MANY_TRY(crdfft::build_roots(t + crdfft::kManyTwLo, crdfft::kManyTwLo, crdfft::kManyTwLo, sz.n, nullptr));
     }
     MANY_TRY(hipStreamSynchronize(nullptr));
     p->workgroups = wgs;
@@ -341,10 +350,77 @@ crdfft::ManyArgs args_of(const crdfft_many_plan_s* p, const void* in, void* out)
                             static_cast<crdfft::cfloat*>(out),
                             static_cast<const crdfft::ManySize*>(p->sizes.p),
                             static_cast<const crdfft::ManyTile*>(p->tiles.p),
-                            static_cast<const uint64_t*>(p->offsets.p),
                             static_cast<const crdfft::cfloat*>(p->tw.p),
                             static_cast<const uint32_t*>(p->wg_begin.p)};
 }
+
+// Median time of `reps` launches (after a warm one) of a mixed plan on its own stream, hipEvents.
+struct Timer
+{
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t init()
+    {
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess)
+        {
+            e = hipEventCreate(&e0);
+        }
+        if (e == hipSuccess)
+        {
+            e = hipEventCreate(&e1);
+        }
+        return e;
+    }
+    ~Timer()
+    {
+        if (e0 != nullptr)
+        {
+            (void)hipEventDestroy(e0);
+        }
+        if (e1 != nullptr)
+        {
+            (void)hipEventDestroy(e1);
+        }
+        if (s != nullptr)
+        {
+            (void)hipStreamDestroy(s);
+        }
+    }
+    crdfft_status time(const uint64_t* lengths, uint64_t count, const crdfft_radix_plan* choices, uint64_t nchoices,
+                       void* in, void* out, uint32_t reps, double* median_ms)
+    {
+        crdfft_many_plan mp = nullptr;
+        crdfft_status st = crdfft_many_plan_create(&mp, lengths, nullptr, count, choices, nchoices, 0);
+        if (st != CRDFFT_OK)
+        {
+            return st;
+        }
+        std::vector<float> ms(reps);
+        st = crdfft_many_execute(mp, in, out, CRDFFT_FORWARD, s); // warm
+        for (uint32_t r = 0; r < reps && st == CRDFFT_OK; ++r)
+        {
+            if (hipEventRecord(e0, s) != hipSuccess)
+            {
+                st = CRDFFT_HIP_ERROR;
+                break;
+            }
+            st = crdfft_many_execute(mp, in, out, CRDFFT_FORWARD, s);
+            if (st == CRDFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                    hipEventElapsedTime(&ms[r], e0, e1) != hipSuccess))
+            {
+                st = CRDFFT_HIP_ERROR;
+            }
+        }
+        crdfft_many_plan_destroy(mp);
+        if (st == CRDFFT_OK)
+        {
+            std::nth_element(ms.begin(), ms.begin() + reps / 2, ms.end());
+            *median_ms = ms[reps / 2];
+        }
+        return st;
+    }
+};
 
 } // namespace
 
@@ -488,8 +564,10 @@ crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint
         return CRDFFT_INVALID_ARGUMENT;
     }
     std::vector<uint64_t> distinct;
+    uint64_t total = 0;
     for (uint64_t t = 0; t < count; ++t)
     {
+        total += lengths[t];
         if (std::find(distinct.begin(), distinct.end(), lengths[t]) == distinct.end())
         {
             distinct.push_back(lengths[t]);
@@ -500,15 +578,13 @@ crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint
         return CRDFFT_INVALID_ARGUMENT;
     }
     reps = std::max<uint32_t>(reps, 1);
-    constexpr uint64_t kPoints = uint64_t{1} << 20;
+    // each candidate runs on 2^24 points of its length: a thousand full tiles, several per CU
+    constexpr uint64_t kPoints = uint64_t{1} << 24;
     DevBuf buf;
     MANY_TRY(buf.alloc(kPoints * 2 * sizeof(crdfft::cfloat)));
     MANY_TRY(hipMemset(buf.p, 0, kPoints * 2 * sizeof(crdfft::cfloat)));
-    hipStream_t s = nullptr;
-    MANY_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    MANY_TRY(hipEventCreate(&e0));
-    MANY_TRY(hipEventCreate(&e1));
+    Timer tm;
+    MANY_TRY(tm.init());
     uint64_t nrec = 0;
     crdfft_status result = CRDFFT_OK;
     for (size_t i = 0; i < distinct.size() && result == CRDFFT_OK; ++i)
@@ -526,37 +602,14 @@ crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint
         crdfft_radix_plan best_plan = cands[0];
         for (size_t c = 0; c < cands.size() && result == CRDFFT_OK; ++c)
         {
-            crdfft_many_plan mp = nullptr;
-            result = crdfft_many_plan_create(&mp, lens.data(), nullptr, tcount, &cands[c], 1, 0);
+            double ms = 0.0;
+            result = tm.time(lens.data(), tcount, &cands[c], 1, buf.p, static_cast<crdfft::cfloat*>(buf.p) + kPoints,
+                             reps, &ms);
             if (result != CRDFFT_OK)
             {
                 break;
             }
-            std::vector<float> ms(reps);
-            void* in = buf.p;
-            void* out = static_cast<crdfft::cfloat*>(buf.p) + kPoints;
-            result = crdfft_many_execute(mp, in, out, CRDFFT_FORWARD, s); // warm
-            for (uint32_t r = 0; r < reps && result == CRDFFT_OK; ++r)
-            {
-                if (hipEventRecord(e0, s) != hipSuccess)
-                {
-                    result = CRDFFT_HIP_ERROR;
-                    break;
-                }
-                result = crdfft_many_execute(mp, in, out, CRDFFT_FORWARD, s);
-                if (result == CRDFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                            hipEventElapsedTime(&ms[r], e0, e1) != hipSuccess))
-                {
-                    result = CRDFFT_HIP_ERROR;
-                }
-            }
-            crdfft_many_plan_destroy(mp);
-            if (result != CRDFFT_OK)
-            {
-                break;
-            }
-            std::nth_element(ms.begin(), ms.begin() + reps / 2, ms.end());
-            const double us = 1000.0 * ms[reps / 2] / static_cast<double>(tcount);
+            const double us = 1000.0 * ms / static_cast<double>(tcount);
             if (records != nullptr && nrec < record_capacity)
             {
                 crdfft_tune_record& rec = records[nrec];
@@ -574,9 +627,34 @@ crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint
         }
         choices[i] = best_plan;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
+    // the per-length winners are measured alone; the batch as given is the judge: if the planner's defaults run it
+    // faster, they are returned instead (skipped when the batch does not fit beside the tuning buffers)
+    if (result == CRDFFT_OK && count > 0)
+    {
+        DevBuf win, wout;
+        if (win.alloc(total * sizeof(crdfft::cfloat)) == hipSuccess &&
+            wout.alloc(total * sizeof(crdfft::cfloat)) == hipSuccess &&
+            hipMemset(win.p, 0, total * sizeof(crdfft::cfloat)) == hipSuccess)
+        {
+            double tuned = 0.0, plain = 0.0;
+            result = tm.time(lengths, count, choices, distinct.size(), win.p, wout.p, reps, &tuned);
+            if (result == CRDFFT_OK)
+            {
+                result = tm.time(lengths, count, nullptr, 0, win.p, wout.p, reps, &plain);
+            }
+            if (result == CRDFFT_OK && plain < tuned)
+            {
+                for (size_t i = 0; i < distinct.size(); ++i)
+                {
+                    choices[i] = candidates(distinct[i])[0];
+                }
+            }
+        }
+        else
+        {
+            (void)hipGetLastError();
+        }
+    }
     *nchoices = distinct.size();
     if (nrecords != nullptr)
     {

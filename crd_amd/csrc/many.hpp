@@ -12,22 +12,39 @@ namespace crdfft
 constexpr int kManyThreads = 1024;      // one workgroup per CU
 constexpr int kManyTilePoints = 16384;  // complex points per tile (LDS: split re/im + pad)
 constexpr int kManyMaxStages = 16;
+constexpr int kManyTwLo = 128;          // twiddle of a length: W_n^e = hi[e >> 7] * lo[e & 127]
+constexpr int kManyTwEntries = 2 * kManyTwLo;
+
+// x / d for x * d < 2^32 as one multiply-high: magic = ceil(2^32 / d), 0 for d == 1.
+inline uint32_t many_magic(uint32_t d) { return d <= 1 ? 0u : static_cast<uint32_t>(((uint64_t{1} << 32) + d - 1) / d); }
+
+// One Stockham stage of a length: radix R, butterflies per transform n / R, sub-DFT size p before the stage, and
+// the stage twiddle W_{pR} = W_n^{twstep}.
+struct ManyStage
+{
+    uint32_t radix;
+    uint32_t nr, nr_magic;
+    uint32_t p, p_magic;
+    uint32_t twstep;
+};
 
 // One distinct length and its radix plan.
 struct ManySize
 {
     uint32_t n;
+    uint32_t n_magic;
     uint32_t nstages;
-    uint32_t radix[kManyMaxStages];
-    uint64_t tw_offset; // W_n^e, e < n, at tw[tw_offset + e]
+    uint32_t reserved;
+    uint64_t tw_offset; // lo[128] then hi[128] at tw[tw_offset ...]
+    ManyStage stage[kManyMaxStages];
 };
 
-// `count` transforms of one length whose element offsets are offsets[first .. first + count).
+// `count` transforms of one length, contiguous at element offset base of both in and out.
 struct ManyTile
 {
     uint32_t size_id;
     uint32_t count;
-    uint64_t first;
+    uint64_t base;
 };
 
 struct ManyArgs
@@ -36,7 +53,6 @@ struct ManyArgs
     cfloat* out;
     const ManySize* sizes;
     const ManyTile* tiles;
-    const uint64_t* offsets;
     const cfloat* tw;
     const uint32_t* wg_begin; // workgroups + 1 entries
 };

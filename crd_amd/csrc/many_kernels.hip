@@ -1,15 +1,23 @@
 // The mixed-size persistent kernel: thousands of C2C transforms of 2/3/5/7-smooth lengths (up to 16384) in one
 // launch.
 //
-// The host (many.cpp) groups the transforms by length into tiles of at most 16384 points (several transforms of one
+// The host (many.cpp) cuts the batch into tiles (runs of at most 16384 points of contiguous transforms of one
 // length) and balances the tiles over a grid of one 1024-thread workgroup per CU (longest-processing-time first), so
 // the kernel has no queue, no atomics and no inter-workgroup wait: workgroup w walks tiles [begin[w], begin[w+1]).
-// Each tile runs the length's radix plan (chosen per length by the autotuner) as a Stockham autosort whose radix is a
-// runtime value dispatched to compile-time butterflies: stage s reads butterfly b of transform j at slots
-// b + m*N/R, twiddles by W_{PR}^{m (b mod P)}, and writes slot (b - b mod P)*R + (b mod P) + m*P. The first stage reads
-// global memory and the last writes it (both unit-stride across lanes); the stages between exchange through LDS (split
-// re/im, one pad float per 16). Radices 3, 5 and 7 are new relative to the reference, which sends every length that
-// is not a power of two to Bluestein (engine/numerics/hesap-fft/include/crd/hesap/fft/bluestein.hpp).
+//
+// Per tile: the tile's points, loaded into registers while the previous tile was transformed, go to LDS in their
+// flat order (transform j, element i at j*n + i); the loads of the next tile are issued; then the length's radix
+// plan (chosen per length by the autotuner) runs as a Stockham autosort whose radix is a runtime value dispatched to
+// compile-time butterflies: stage s reads butterfly b of transform j at slots b + m*n/R, twiddles by
+// W_{pR}^{m (b mod p)}, and writes slot (b - b mod p)*R + (b mod p) + m*p, into LDS or, in the last stage, to
+// global memory. Stages touch no global memory but the last one's stores, so the next tile's loads (which complete
+// in issue order with every other vector memory access) are not waited for until the tile is done. Twiddles come
+// from a 256-entry LDS table per length, W_n^e = hi[e >> 7] * lo[e & 127] (both FP64-built), and the index
+// divisions are multiply-highs by precomputed magic numbers.
+//
+// Radices 3, 5 and 7 are new relative to the reference, which sends every length that is not a power of two to
+// Bluestein (engine/numerics/hesap-fft/include/crd/hesap/fft/bluestein.hpp).
+#include "buffer.hpp"
 #include "dft.hpp"
 #include "fft_core.hpp"
 #include "many.hpp"
@@ -23,97 +31,117 @@ namespace
 
 constexpr int kThreads = kManyThreads;
 constexpr int kTilePoints = kManyTilePoints;
+constexpr int kPer = kTilePoints / kThreads; // tile points per thread
 constexpr int kLdsFloats = kTilePoints + (kTilePoints >> 4) + 16;
+
+// x / d by the host's magic number (exact for x * d < 2^32).
+__device__ __forceinline__ uint32_t mdiv(uint32_t x, uint32_t magic) { return magic == 0 ? x : __umulhi(x, magic); }
+
+// The plan's read-only tables through the constant address space: their uniform reads become scalar loads, which
+// complete on their own counter instead of waiting behind the prefetched tile's vector loads.
+template <typename T> using cptr = const __attribute__((address_space(4))) T*;
+template <typename T> __device__ __forceinline__ cptr<T> constant(const T* p)
+{
+    return reinterpret_cast<cptr<T>>(reinterpret_cast<uintptr_t>(p));
+}
+
+__device__ __forceinline__ ManyTile read(cptr<ManyTile> p)
+{
+    ManyTile t;
+    t.size_id = p->size_id;
+    t.count = p->count;
+    t.base = p->base;
+    return t;
+}
+
+__device__ __forceinline__ ManyStage read(cptr<ManyStage> p)
+{
+    ManyStage s;
+    s.radix = p->radix;
+    s.nr = p->nr;
+    s.nr_magic = p->nr_magic;
+    s.p = p->p;
+    s.p_magic = p->p_magic;
+    s.twstep = p->twstep;
+    return s;
+}
 
 struct TileCtx
 {
-    const cf* in;
-    cf* out;
-    const uint64_t* offsets; // element offsets of the tile's transforms (in and out share them)
-    const cf* tw;            // W_N^e, e < N
+    __amdgpu_buffer_rsrc_t out; // the tile's output run
     float* lre;
     float* lim;
+    const cf* tlo; // LDS: W_n^j, j < 128
+    const cf* thi; // LDS: W_n^{128 i}
     uint32_t n;
     uint32_t count;
 };
 
 template <int R, bool INV>
-__device__ __noinline__ void many_stage(const TileCtx& c, uint32_t p, bool first, bool last, int tid)
+__device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st, bool last, int tid)
 {
-    constexpr int QMAX = (kTilePoints / kThreads + R - 1) / R;
-    const uint32_t nr = c.n / R;
+    constexpr int Q = (kPer + R - 1) / R;
+    const uint32_t nr = st.nr;
     const uint32_t total = c.count * nr;
-    const uint32_t twstep = c.n / (p * R); // W_{PR} = W_N^{twstep}
-    cf a[QMAX][R];
+    cf a[Q][R];
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q)
+    for (int q = 0; q < Q; ++q)
     {
         const uint32_t g = tid + q * kThreads;
         if (g < total)
         {
-            const uint32_t j = g / nr;
-            const uint32_t b = g - j * nr;
-            if (first)
-            {
-                const cf* src = c.in + c.offsets[j];
+            const uint32_t j = mdiv(g, st.nr_magic);
+            const uint32_t base = g + j * (c.n - nr); // j * n + b
 #pragma unroll
-                for (int m = 0; m < R; ++m)
-                {
-                    a[q][m] = src[b + m * nr];
-                }
-            }
-            else
+            for (int m = 0; m < R; ++m)
             {
-                const uint32_t base = j * c.n + b;
-#pragma unroll
-                for (int m = 0; m < R; ++m)
-                {
-                    const int o = pad16(static_cast<int>(base + m * nr));
-                    a[q][m] = cf{c.lre[o], c.lim[o]};
-                }
+                const int o = pad16(static_cast<int>(base + m * nr));
+                a[q][m] = cf{c.lre[o], c.lim[o]};
             }
         }
     }
-    if (!first)
-    {
-        __syncthreads(); // every read of this stage precedes any write into the same LDS buffer
-    }
+    __syncthreads(); // every read of this stage precedes any write into the same LDS buffer
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q)
+    for (int q = 0; q < Q; ++q)
     {
         const uint32_t g = tid + q * kThreads;
         if (g < total)
         {
-            const uint32_t j = g / nr;
+            const uint32_t j = mdiv(g, st.nr_magic);
             const uint32_t b = g - j * nr;
-            const uint32_t k = b % p;
-            if (p > 1)
+            const uint32_t k = b - mdiv(b, st.p_magic) * st.p;
+            if (st.p > 1)
             {
-                const uint32_t ek = k * twstep;
+                const uint32_t e = k * st.twstep; // W_{pR}^{k} = W_n^{e}
 #pragma unroll
                 for (int m = 1; m < R; ++m)
                 {
-                    a[q][m] = ctw<INV>(a[q][m], c.tw[m * ek]);
+                    const uint32_t em = e * m;
+                    a[q][m] = ctw<INV>(a[q][m], cmul(c.thi[em >> 7], c.tlo[em & (kManyTwLo - 1)]));
+                    if ((m & 3) == 3)
+                    {
+                        // keep the scheduler from hoisting every table read of the butterfly at once (register
+                        // pressure: the next tile's points are live across the stage)
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
             }
             Dft<R, INV>::run(a[q]);
-            const uint32_t ob = (b - k) * R + k;
+            const uint32_t ob = j * c.n + (b - k) * R + k;
             if (last)
             {
-                cf* dst = c.out + c.offsets[j];
 #pragma unroll
                 for (int m = 0; m < R; ++m)
                 {
-                    dst[ob + m * p] = a[q][m];
+                    bst<kAuxNt>(c.out, (ob + m * st.p) * 8u, 0, a[q][m]);
                 }
             }
             else
             {
-                const uint32_t base = j * c.n + ob;
 #pragma unroll
                 for (int m = 0; m < R; ++m)
                 {
-                    const int o = pad16(static_cast<int>(base + m * p));
+                    const int o = pad16(static_cast<int>(ob + m * st.p));
                     c.lre[o] = a[q][m].x;
                     c.lim[o] = a[q][m].y;
                 }
@@ -126,45 +154,97 @@ __device__ __noinline__ void many_stage(const TileCtx& c, uint32_t p, bool first
     }
 }
 
+// The loads of one tile into registers: point f = tid + k * threads, and (tid < 256) one twiddle entry.
+__device__ __forceinline__ void fetch_tile(const ManyArgs& a, uint32_t t, int tid, cf (&pf)[kPer], cf& ptw)
+{
+    const ManyTile tile = read(constant(a.tiles) + t);
+    const uint32_t n = constant(a.sizes)[tile.size_id].n;
+    const uint64_t tw_offset = constant(a.sizes)[tile.size_id].tw_offset;
+    const uint32_t points = tile.count * n;
+    const __amdgpu_buffer_rsrc_t src = make_rsrc(a.in + tile.base, points * 8u);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+    {
+        const uint32_t f = tid + k * kThreads;
+        pf[k] = f < points ? bld<kAuxNt>(src, f * 8u, 0) : cf{0.0f, 0.0f};
+    }
+    if (tid < kManyTwEntries)
+    {
+        const cfloat w = a.tw[tw_offset + tid];
+        ptw = cf{w.x, w.y};
+    }
+}
+
 template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs a)
 {
     extern __shared__ float lds[];
+    float* lre = lds;
+    float* lim = lds + kLdsFloats;
+    cf* ltw = reinterpret_cast<cf*>(lds + 2 * kLdsFloats);
     const int tid = threadIdx.x;
-    const uint32_t wg = blockIdx.x;
-    const uint32_t t0 = a.wg_begin[wg];
-    const uint32_t t1 = a.wg_begin[wg + 1];
+    const uint32_t t0 = constant(a.wg_begin)[blockIdx.x];
+    const uint32_t t1 = constant(a.wg_begin)[blockIdx.x + 1];
+    // LDS slot of point tid + k * threads: pad16 of it is o0 + k * (threads + threads / 16)
+    const int o0 = pad16(tid);
+    constexpr int kStep = kThreads + kThreads / 16;
+    cf pf[kPer];
+    cf ptw = cf{1.0f, 0.0f};
+    if (t0 < t1)
+    {
+        fetch_tile(a, t0, tid, pf, ptw);
+    }
+#pragma unroll 1
     for (uint32_t t = t0; t < t1; ++t)
     {
-        const ManyTile tile = a.tiles[t];
-        const ManySize& sz = a.sizes[tile.size_id];
-        TileCtx c{reinterpret_cast<const cf*>(a.in), reinterpret_cast<cf*>(a.out), a.offsets + tile.first,
-                  reinterpret_cast<const cf*>(a.tw) + sz.tw_offset, lds, lds + kLdsFloats, sz.n, tile.count};
-        uint32_t p = 1;
-        const uint32_t ns = sz.nstages;
+        const ManyTile tile = read(constant(a.tiles) + t);
+        cptr<ManySize> sz = constant(a.sizes) + tile.size_id;
+        const uint32_t n = sz->n;
+        const uint32_t points = tile.count * n;
+        __syncthreads(); // the previous tile's last LDS reads are done
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const uint32_t f = tid + k * kThreads;
+            if (f < points)
+            {
+                lre[o0 + k * kStep] = pf[k].x;
+                lim[o0 + k * kStep] = pf[k].y;
+            }
+        }
+        if (tid < kManyTwEntries)
+        {
+            ltw[tid] = ptw;
+        }
+        __syncthreads();
+        if (t + 1 < t1)
+        {
+            fetch_tile(a, t + 1, tid, pf, ptw);
+        }
+        const TileCtx c{make_rsrc(a.out + tile.base, points * 8u), lre, lim, ltw, ltw + kManyTwLo, n, tile.count};
+        const uint32_t ns = sz->nstages;
+#pragma unroll 1
         for (uint32_t s = 0; s < ns; ++s)
         {
-            const uint32_t r = sz.radix[s];
-            const bool first = s == 0;
+            const ManyStage st = read(&sz->stage[s]);
             const bool last = s + 1 == ns;
-            switch (r)
+            switch (st.radix)
             {
-                case 1: many_stage<1, INV>(c, p, first, last, tid); break;
-                case 2: many_stage<2, INV>(c, p, first, last, tid); break;
-                case 3: many_stage<3, INV>(c, p, first, last, tid); break;
-                case 4: many_stage<4, INV>(c, p, first, last, tid); break;
-                case 5: many_stage<5, INV>(c, p, first, last, tid); break;
-                case 7: many_stage<7, INV>(c, p, first, last, tid); break;
-                case 8: many_stage<8, INV>(c, p, first, last, tid); break;
-                default: many_stage<16, INV>(c, p, first, last, tid); break;
+                case 1: many_stage<1, INV>(c, st, last, tid); break;
+                case 2: many_stage<2, INV>(c, st, last, tid); break;
+                case 3: many_stage<3, INV>(c, st, last, tid); break;
+                case 4: many_stage<4, INV>(c, st, last, tid); break;
+                case 5: many_stage<5, INV>(c, st, last, tid); break;
+                case 7: many_stage<7, INV>(c, st, last, tid); break;
+                case 8: many_stage<8, INV>(c, st, last, tid); break;
+                default: many_stage<16, INV>(c, st, last, tid); break;
             }
-            p *= r;
         }
     }
 }
 
 } // namespace
 
-unsigned many_lds_bytes() { return 2u * kLdsFloats * 4u; }
+unsigned many_lds_bytes() { return (2u * kLdsFloats + 2u * kManyTwEntries) * 4u; }
 
 hipError_t many_prepare()
 {

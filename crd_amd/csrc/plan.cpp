@@ -232,11 +232,12 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
             return CRDFFT_UNSUPPORTED;
         }
         // (the planner's count assumed 256 CUs and may also carry a tuned cap)
+        const uint64_t planned = sh.chunk;
         p->shape.chunk = std::min<uint64_t>(static_cast<uint64_t>(d.multiProcessorCount) * per_cu,
                                             std::max<uint64_t>(sh.batch, 1));
-        if (sh.chunk != 0 && sh.chunk < p->shape.chunk)
+        if (planned != 0 && planned < p->shape.chunk)
         {
-            p->shape.chunk = sh.chunk;
+            p->shape.chunk = planned;
         }
         p->shape.workspace_bytes = p->shape.chunk * cn * sizeof(cfloat);
         CRDFFT_TRY(p->workspace.alloc(p->shape.workspace_bytes));
@@ -318,7 +319,14 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                    cn,
                                    sh.hints & crdfft::kHintNtLoad};
                 CRDFFT_TRY(prof.before());
-                CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
+                if ((sh.hints & crdfft::kHintPersistent) != 0)
+                {
+                    CRDFFT_TRY(crdfft::launch_cols_pf(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
+                }
+                else
+                {
+                    CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
+                }
                 CRDFFT_TRY(prof.after(0, cn * cnt));
                 crdfft::RowTrArgs ra{p->workspace.as<cfloat>(), out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
                                      cn, odist, scale, sh.hints & crdfft::kHintNtStore};

@@ -269,7 +269,12 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
         s.hints = kDefaultFourStepHints;
         if (tuning != nullptr && tuning->hints != 0)
         {
-            s.hints = tuning->hints & 3u;
+            s.hints = tuning->hints & 11u;
+        }
+        if ((s.hints & kHintPersistentColumns) != 0 &&
+            (!have_cols_pf(s.n1, s.e1, s.tile1) || (s.n2 % static_cast<uint64_t>(s.tile1)) != 0))
+        {
+            return CRDFFT_UNSUPPORTED;
         }
     }
     *out = s;

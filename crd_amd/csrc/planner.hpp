@@ -40,8 +40,10 @@ constexpr uint64_t kFourStepMax = uint64_t{1} << 24;
 // launch gaps and the partial last wave cost little (measured on MI355X: 64 MiB chunks ran 15% slower than 128 MiB
 // and those slower than 256 MiB at n = 2^16; the intermediate does not measurably stay in the Infinity Cache).
 constexpr uint64_t kChunkBytes = uint64_t{256} << 20;
-// Cache hints of the four-step: bit 0 streams the column pass's input past the caches, bit 1 the row pass's output.
+// Plan hints of the four-step: bit 0 streams the column pass's input past the caches, bit 1 the row pass's output;
+// bit 3 runs the column pass as the persistent prefetching launch.
 constexpr uint32_t kDefaultFourStepHints = 3;
+constexpr uint32_t kHintPersistentColumns = 8;
 // The largest complex length of the fused four-step (one transform's intermediate per workgroup slot).
 constexpr uint64_t kFusedMax = uint64_t{1} << 17;
 // Workgroup size of the fused four-step unless tuned: 512 threads (two workgroups per CU hide each other's memory
@@ -67,6 +69,7 @@ bool element_major_shape(uint64_t n, uint64_t batch, int* e, int* c);
 bool have_row_kernel(uint64_t n, int e);
 bool have_col_kernel(uint64_t n1, int e, int c);
 bool have_rowtr_kernel(uint64_t n2, int e, int r);
+bool have_cols_pf(uint64_t n1, int e, int c);
 // Fused four-step of n1 x n2 at threads per workgroup; lds_bytes receives its LDS per workgroup.
 bool have_fused_kernel(uint64_t n1, int e1, uint64_t n2, int e2, int threads, unsigned* lds_bytes = nullptr);
 

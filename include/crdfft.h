@@ -241,9 +241,11 @@ typedef struct crdfft_tune_record
 } crdfft_tune_record;
 
 /* The autotuner of the mixed-size path: for every distinct length of `lengths`, times every candidate radix plan
- * (crdfft_radix_candidates) in the persistent kernel on a synthetic batch of at least 2^20 points (median of `reps`
- * launches, hipEvents) and writes the fastest plan per length to choices (capacity >= distinct lengths). Every
- * measurement is appended to records when records != NULL (at most record_capacity). Synchronous. */
+ * (crdfft_radix_candidates) in the persistent kernel on a synthetic batch of 2^24 points of that length (median of
+ * `reps` launches, hipEvents) and writes the fastest plan per length to choices (capacity >= distinct lengths). The
+ * batch as given is then timed with those plans and with the planner's defaults, and the defaults are returned if
+ * they run it faster. Every per-length measurement is appended to records when records != NULL (at most
+ * record_capacity). Synchronous. */
 CRDFFT_API crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint32_t reps,
                                               crdfft_radix_plan* choices, uint64_t capacity, uint64_t* nchoices,
                                               crdfft_tune_record* records, uint64_t record_capacity,

@@ -140,3 +140,22 @@ def test_fused_real_transforms(gpu, po):
     gpu.synchronize()
     y = back.download(xr.shape, np.float32)
     assert np.max(np.abs(y - xr)) <= bound(n) * np.max(np.abs(xr))
+
+
+@pytest.mark.parametrize("n,batch,n1,tile1", [(1 << 17, 5, 256, 16), (1 << 18, 3, 512, 16),
+                                             (1 << 20, 3, 1024, 8), (1 << 20, 3, 2048, 8),
+                                             (1 << 20, 2, 1024, 16), (1 << 21, 2, 2048, 8)])
+def test_persistent_column_pass(gpu, po, n, batch, n1, tile1):
+    """The column pass as a persistent prefetching launch (plan hint bit 3) gives the results of the one-tile-per-
+    workgroup column pass, for full-table (n <= 2^18) and factored twiddles."""
+    x = po.lcg_complex(n * batch, n ^ 0x5EED).reshape(batch, n)
+    base = {"algorithm": 2, "n1": n1, "tile1": tile1}
+    pf = gpu.FftPlan(n, batch, tuning=dict(base, hints=11))
+    assert pf.info()["hints"] & 8
+    plain = gpu.FftPlan(n, batch, tuning=dict(base, hints=3))
+    for inverse in (False, True):
+        y = run(gpu, pf, x, inverse)
+        z = run(gpu, plain, x, inverse)
+        ref = po.fft64_of_f32(x, inverse=inverse)
+        assert po.rel_l2(y, ref) <= bound(n), f"n={n} inverse={inverse}"
+        assert po.rel_l2(y, z) <= 1e-7, f"n={n} inverse={inverse} persistent vs one tile per workgroup"

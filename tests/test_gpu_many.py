@@ -127,18 +127,24 @@ def test_graph_replays_match_eager(gpu, po):
 
 
 def test_autotune_picks_valid_fastest_plans(gpu):
+    """Every length gets its fastest timed candidate, unless the planner's defaults ran the whole batch faster, in
+    which case every length gets its default (candidate 0)."""
     from crd_amd import many
 
     lengths = [64, 1000, 1000, 4096, 15876]
     choices, records = many.autotune(lengths, reps=3)
     assert sorted(int(c.n) for c in choices) == sorted(set(lengths))
+    fastest, default = [], []
     for c in choices:
         assert int(np.prod(c.radices())) == c.n
         mine = [r for r in records if r.plan.n == c.n]
         assert len(mine) == len(many.radix_candidates(int(c.n)))
-        best = min(mine, key=lambda r: r.us_per_transform)
-        assert best.plan.radices() == c.radices() and best.plan.group == c.group
         assert all(r.us_per_transform > 0 for r in mine)
+        best = min(mine, key=lambda r: r.us_per_transform)
+        first = many.radix_candidates(int(c.n))[0]
+        fastest.append(best.plan.radices() == c.radices() and best.plan.group == c.group)
+        default.append(first.radices() == c.radices() and first.group == c.group)
+    assert all(fastest) or all(default), (fastest, default)
 
 
 def test_rejects_non_smooth_lengths(gpu):
