@@ -74,8 +74,62 @@ bool smooth(uint64_t n, Exponents* e)
 // Transforms of length n in one full tile.
 uint32_t max_group(uint64_t n) { return static_cast<uint32_t>(crdfft::kManyTilePoints / n); }
 
-// Candidates: the power-of-two part in radices of at most `cap` (16, 8, 4; one smaller radix for the remaining
-// bits), the odd part as 7s, 5s and 3s; powers of two first or odd radices first; full tiles or half tiles.
+// The radices of one candidate: the power-of-two part in radices of at most 2^capbits (one smaller radix first for
+// the remaining bits), the odd part as 7s, 5s and 3s; merged, pairs of 3s become 9s and a leftover 3 or 5 takes a
+// 2 or 4 of the power-of-two part (6, 12, 10), so the length runs in fewer stages.
+void candidate_radices(const Exponents& e, uint32_t capbits, bool merge, std::vector<uint32_t>* p2_out,
+                       std::vector<uint32_t>* odd_out)
+{
+    std::vector<uint32_t> p2;
+    uint32_t a = e.a;
+    while (a >= capbits)
+    {
+        p2.push_back(1u << capbits);
+        a -= capbits;
+    }
+    if (a > 0)
+    {
+        p2.insert(p2.begin(), 1u << a);
+    }
+    std::vector<uint32_t> odd;
+    uint32_t b = e.b, c = e.c, d = e.d;
+    if (merge)
+    {
+        for (; b >= 2; b -= 2)
+        {
+            odd.push_back(9u);
+        }
+    }
+    std::vector<uint32_t> singles;
+    singles.insert(singles.end(), d, 7u);
+    singles.insert(singles.end(), c, 5u);
+    singles.insert(singles.end(), b, 3u);
+    for (uint32_t r : singles)
+    {
+        bool done = false;
+        if (merge)
+        {
+            for (uint32_t& q : p2)
+            {
+                if ((q == 2 && r != 7) || (q == 4 && r == 3))
+                {
+                    q *= r; // 6, 10, 12
+                    done = true;
+                    break;
+                }
+            }
+        }
+        if (!done)
+        {
+            odd.push_back(r);
+        }
+    }
+    *p2_out = p2;
+    *odd_out = odd;
+}
+
+// Candidates: merged and unmerged radices, capbits 4, 3, 2, the power-of-two part first or the odd part first, full
+// tiles or half tiles (candidate 0, the planner's default: merged, radix 16, powers of two first, full tiles).
 std::vector<crdfft_radix_plan> candidates(uint64_t n)
 {
     std::vector<crdfft_radix_plan> out;
@@ -85,54 +139,44 @@ std::vector<crdfft_radix_plan> candidates(uint64_t n)
         return out;
     }
     const uint32_t full = max_group(n);
-    for (uint32_t capbits : {4u, 3u, 2u})
+    for (int merge = 1; merge >= 0; --merge)
     {
-        std::vector<uint32_t> p2;
-        uint32_t a = e.a;
-        while (a >= capbits)
+        for (uint32_t capbits : {4u, 3u, 2u})
         {
-            p2.push_back(1u << capbits);
-            a -= capbits;
-        }
-        if (a > 0)
-        {
-            p2.insert(p2.begin(), 1u << a);
-        }
-        std::vector<uint32_t> odd;
-        odd.insert(odd.end(), e.d, 7u);
-        odd.insert(odd.end(), e.c, 5u);
-        odd.insert(odd.end(), e.b, 3u);
-        for (int order = 0; order < 2; ++order)
-        {
-            std::vector<uint32_t> r = order == 0 ? p2 : odd;
-            const std::vector<uint32_t>& tail = order == 0 ? odd : p2;
-            r.insert(r.end(), tail.begin(), tail.end());
-            if (r.empty())
+            std::vector<uint32_t> p2, odd;
+            candidate_radices(e, capbits, merge != 0, &p2, &odd);
+            for (int order = 0; order < 2; ++order)
             {
-                r.push_back(1u); // n == 1
-            }
-            if (r.size() > static_cast<size_t>(crdfft::kManyMaxStages))
-            {
-                continue;
-            }
-            for (uint32_t group : {full, full / 2})
-            {
-                if (group == 0 || (group != full && group == full / 2 && full / 2 == full))
+                std::vector<uint32_t> r = order == 0 ? p2 : odd;
+                const std::vector<uint32_t>& tail = order == 0 ? odd : p2;
+                r.insert(r.end(), tail.begin(), tail.end());
+                if (r.empty())
+                {
+                    r.push_back(1u); // n == 1
+                }
+                if (r.size() > static_cast<size_t>(crdfft::kManyMaxStages))
                 {
                     continue;
                 }
-                crdfft_radix_plan p{};
-                p.n = n;
-                p.nstages = static_cast<uint32_t>(r.size());
-                std::copy(r.begin(), r.end(), p.radix);
-                p.group = group;
-                const bool dup = std::any_of(out.begin(), out.end(), [&](const crdfft_radix_plan& q) {
-                    return q.nstages == p.nstages && q.group == p.group &&
-                           std::equal(q.radix, q.radix + q.nstages, p.radix);
-                });
-                if (!dup)
+                for (uint32_t group : {full, full / 2})
                 {
-                    out.push_back(p);
+                    if (group == 0 || (group != full && group == full / 2 && full / 2 == full))
+                    {
+                        continue;
+                    }
+                    crdfft_radix_plan p{};
+                    p.n = n;
+                    p.nstages = static_cast<uint32_t>(r.size());
+                    std::copy(r.begin(), r.end(), p.radix);
+                    p.group = group;
+                    const bool dup = std::any_of(out.begin(), out.end(), [&](const crdfft_radix_plan& q) {
+                        return q.nstages == p.nstages && q.group == p.group &&
+                               std::equal(q.radix, q.radix + q.nstages, p.radix);
+                    });
+                    if (!dup)
+                    {
+                        out.push_back(p);
+                    }
                 }
             }
         }
@@ -218,7 +262,7 @@ crdfft_status build_many(crdfft_many_plan_s* p, const uint64_t* lengths, const u
             for (uint32_t s = 0; s < chosen.nstages && s < static_cast<uint32_t>(crdfft::kManyMaxStages); ++s)
             {
                 const uint32_t r = chosen.radix[s];
-                if (r != 1 && r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16)
+                if (!crdfft::many_radix_supported(r))
                 {
                     return CRDFFT_INVALID_ARGUMENT;
                 }

@@ -10,10 +10,23 @@ namespace crdfft
 {
 
 constexpr int kManyThreads = 1024;      // one workgroup per CU
-constexpr int kManyTilePoints = 16384;  // complex points per tile (LDS: split re/im + pad)
+constexpr int kManyTilePoints = 16384;  // complex points per tile (LDS: interleaved complex + pad)
 constexpr int kManyMaxStages = 16;
 constexpr int kManyTwLo = 128;          // twiddle of a length: W_n^e = hi[e >> 7] * lo[e & 127]
 constexpr int kManyTwEntries = 2 * kManyTwLo;
+
+// The radices the kernel has butterflies for (radices 14 and 15 would need more than the 128 VGPRs of a
+// 1024-thread workgroup beside the prefetched tile).
+inline bool many_radix_supported(uint32_t r)
+{
+    switch (r)
+    {
+        case 1: case 2: case 3: case 4: case 5: case 6: case 7: case 8: case 9: case 10: case 12: case 16:
+            return true;
+        default:
+            return false;
+    }
+}
 
 // x / d for x * d < 2^32 as one multiply-high: magic = ceil(2^32 / d), 0 for d == 1.
 inline uint32_t many_magic(uint32_t d) { return d <= 1 ? 0u : static_cast<uint32_t>(((uint64_t{1} << 32) + d - 1) / d); }

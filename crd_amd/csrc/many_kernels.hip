@@ -13,9 +13,10 @@
 // global memory. Stages touch no global memory but the last one's stores, so the next tile's loads (which complete
 // in issue order with every other vector memory access) are not waited for until the tile is done. Twiddles come
 // from a 256-entry LDS table per length, W_n^e = hi[e >> 7] * lo[e & 127] (both FP64-built), and the index
-// divisions are multiply-highs by precomputed magic numbers.
+// divisions are multiply-highs by precomputed magic numbers. A butterfly looks up W_{pR}^k once and forms the
+// other R - 2 powers by a product tree; the LDS holds the tile as interleaved complex slots, one pad slot per 16.
 //
-// Radices 3, 5 and 7 are new relative to the reference, which sends every length that is not a power of two to
+// Radices 3, 5, 7 and the products 6, 9, 10, 12 are new relative to the reference, which sends every length that is not a power of two to
 // Bluestein (engine/numerics/hesap-fft/include/crd/hesap/fft/bluestein.hpp).
 #include "buffer.hpp"
 #include "dft.hpp"
@@ -32,7 +33,7 @@ namespace
 constexpr int kThreads = kManyThreads;
 constexpr int kTilePoints = kManyTilePoints;
 constexpr int kPer = kTilePoints / kThreads; // tile points per thread
-constexpr int kLdsFloats = kTilePoints + (kTilePoints >> 4) + 16;
+constexpr int kLdsPoints = kTilePoints + (kTilePoints >> 4) + 16; // complex slots, one pad per 16
 
 // x / d by the host's magic number (exact for x * d < 2^32).
 __device__ __forceinline__ uint32_t mdiv(uint32_t x, uint32_t magic) { return magic == 0 ? x : __umulhi(x, magic); }
@@ -69,13 +70,86 @@ __device__ __forceinline__ ManyStage read(cptr<ManyStage> p)
 struct TileCtx
 {
     __amdgpu_buffer_rsrc_t out; // the tile's output run
-    float* lre;
-    float* lim;
+    cf* ld;        // LDS: the tile, point o at ld[pad16(o)]
     const cf* tlo; // LDS: W_n^j, j < 128
     const cf* thi; // LDS: W_n^{128 i}
     uint32_t n;
     uint32_t count;
 };
+
+// w[m] = w[1]^m for 2 <= m < R by a product tree (at most four products deep past w[1]).
+template <int R> __device__ __forceinline__ void twiddle_powers(cf (&w)[R])
+{
+    if constexpr (R > 2)
+        w[2] = cmul(w[1], w[1]);
+    if constexpr (R > 3)
+        w[3] = cmul(w[2], w[1]);
+    if constexpr (R > 4)
+        w[4] = cmul(w[2], w[2]);
+    if constexpr (R > 5)
+        w[5] = cmul(w[4], w[1]);
+    if constexpr (R > 6)
+        w[6] = cmul(w[3], w[3]);
+    if constexpr (R > 7)
+        w[7] = cmul(w[4], w[3]);
+    if constexpr (R > 8)
+    {
+        w[8] = cmul(w[4], w[4]);
+#pragma unroll
+        for (int m = 9; m < R; ++m)
+        {
+            w[m] = cmul(w[8], w[m - 8]);
+        }
+    }
+}
+
+// v[m] = point (base + m * stride) of the tile for m < R. When the stride is a multiple of 16 the padded slots are
+// evenly spaced too (pad16(o + 16 s) = pad16(o) + 17 s): one address per butterfly and a scalar step.
+template <int R>
+__device__ __forceinline__ void lds_gather(cf (&v)[R], const cf* ld, uint32_t base, uint32_t stride)
+{
+    if ((stride & 15u) == 0)
+    {
+        const cf* p0 = ld + pad16(static_cast<int>(base));
+        const uint32_t step = stride + (stride >> 4);
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+        {
+            v[m] = p0[m * step];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+        {
+            v[m] = ld[pad16(static_cast<int>(base + m * stride))];
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void lds_scatter(const cf (&v)[R], cf* ld, uint32_t base, uint32_t stride)
+{
+    if ((stride & 15u) == 0)
+    {
+        cf* p0 = ld + pad16(static_cast<int>(base));
+        const uint32_t step = stride + (stride >> 4);
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+        {
+            p0[m * step] = v[m];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+        {
+            ld[pad16(static_cast<int>(base + m * stride))] = v[m];
+        }
+    }
+}
 
 template <int R, bool INV>
 __device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st, bool last, int tid)
@@ -91,13 +165,7 @@ __device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st
         if (g < total)
         {
             const uint32_t j = mdiv(g, st.nr_magic);
-            const uint32_t base = g + j * (c.n - nr); // j * n + b
-#pragma unroll
-            for (int m = 0; m < R; ++m)
-            {
-                const int o = pad16(static_cast<int>(base + m * nr));
-                a[q][m] = cf{c.lre[o], c.lim[o]};
-            }
+            lds_gather<R>(a[q], c.ld, g + j * (c.n - nr), nr); // j * n + b
         }
     }
     __syncthreads(); // every read of this stage precedes any write into the same LDS buffer
@@ -113,16 +181,26 @@ __device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st
             if (st.p > 1)
             {
                 const uint32_t e = k * st.twstep; // W_{pR}^{k} = W_n^{e}
-#pragma unroll
-                for (int m = 1; m < R; ++m)
+                if constexpr (R <= 8 || R == 16)
                 {
-                    const uint32_t em = e * m;
-                    a[q][m] = ctw<INV>(a[q][m], cmul(c.thi[em >> 7], c.tlo[em & (kManyTwLo - 1)]));
-                    if ((m & 3) == 3)
+                    cf w[R];
+                    w[1] = cmul(c.thi[e >> 7], c.tlo[e & (kManyTwLo - 1)]);
+                    twiddle_powers<R>(w);
+#pragma unroll
+                    for (int m = 1; m < R; ++m)
                     {
-                        // keep the scheduler from hoisting every table read of the butterfly at once (register
-                        // pressure: the next tile's points are live across the stage)
-                        __builtin_amdgcn_sched_barrier(0);
+                        a[q][m] = ctw<INV>(a[q][m], w[m]);
+                    }
+                }
+                else
+                {
+                    // two butterflies of 9 to 12 values per thread: one table product per value keeps the registers
+                    // for the data
+#pragma unroll
+                    for (int m = 1; m < R; ++m)
+                    {
+                        const uint32_t em = e * m;
+                        a[q][m] = ctw<INV>(a[q][m], cmul(c.thi[em >> 7], c.tlo[em & (kManyTwLo - 1)]));
                     }
                 }
             }
@@ -138,13 +216,7 @@ __device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st
             }
             else
             {
-#pragma unroll
-                for (int m = 0; m < R; ++m)
-                {
-                    const int o = pad16(static_cast<int>(ob + m * st.p));
-                    c.lre[o] = a[q][m].x;
-                    c.lim[o] = a[q][m].y;
-                }
+                lds_scatter<R>(a[q], c.ld, ob, st.p);
             }
         }
     }
@@ -178,9 +250,8 @@ __device__ __forceinline__ void fetch_tile(const ManyArgs& a, uint32_t t, int ti
 template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs a)
 {
     extern __shared__ float lds[];
-    float* lre = lds;
-    float* lim = lds + kLdsFloats;
-    cf* ltw = reinterpret_cast<cf*>(lds + 2 * kLdsFloats);
+    cf* ld = reinterpret_cast<cf*>(lds);
+    cf* ltw = ld + kLdsPoints;
     const int tid = threadIdx.x;
     const uint32_t t0 = constant(a.wg_begin)[blockIdx.x];
     const uint32_t t1 = constant(a.wg_begin)[blockIdx.x + 1];
@@ -207,8 +278,7 @@ template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs 
             const uint32_t f = tid + k * kThreads;
             if (f < points)
             {
-                lre[o0 + k * kStep] = pf[k].x;
-                lim[o0 + k * kStep] = pf[k].y;
+                ld[o0 + k * kStep] = pf[k];
             }
         }
         if (tid < kManyTwEntries)
@@ -220,7 +290,7 @@ template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs 
         {
             fetch_tile(a, t + 1, tid, pf, ptw);
         }
-        const TileCtx c{make_rsrc(a.out + tile.base, points * 8u), lre, lim, ltw, ltw + kManyTwLo, n, tile.count};
+        const TileCtx c{make_rsrc(a.out + tile.base, points * 8u), ld, ltw, ltw + kManyTwLo, n, tile.count};
         const uint32_t ns = sz->nstages;
 #pragma unroll 1
         for (uint32_t s = 0; s < ns; ++s)
@@ -236,6 +306,10 @@ template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs 
                 case 5: many_stage<5, INV>(c, st, last, tid); break;
                 case 7: many_stage<7, INV>(c, st, last, tid); break;
                 case 8: many_stage<8, INV>(c, st, last, tid); break;
+                case 6: many_stage<6, INV>(c, st, last, tid); break;
+                case 9: many_stage<9, INV>(c, st, last, tid); break;
+                case 10: many_stage<10, INV>(c, st, last, tid); break;
+                case 12: many_stage<12, INV>(c, st, last, tid); break;
                 default: many_stage<16, INV>(c, st, last, tid); break;
             }
         }
@@ -244,7 +318,7 @@ template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs 
 
 } // namespace
 
-unsigned many_lds_bytes() { return (2u * kLdsFloats + 2u * kManyTwEntries) * 4u; }
+unsigned many_lds_bytes() { return (kLdsPoints + kManyTwEntries) * 8u; }
 
 hipError_t many_prepare()
 {

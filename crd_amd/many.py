@@ -77,7 +77,7 @@ def autotune(lengths, reps=5):
     distinct = len(set(int(x) for x in arr))
     choices = (RadixPlan * distinct)()
     nch = ctypes.c_uint64(0)
-    cap = distinct * 16
+    cap = distinct * 32
     recs = (TuneRecord * cap)()
     nrec = ctypes.c_uint64(0)
     check(lib().crdfft_many_autotune(lp, len(arr), reps, choices, distinct, ctypes.byref(nch), recs, cap,

@@ -187,7 +187,7 @@ CRDFFT_API crdfft_status crdfft_nd_plan_destroy(crdfft_nd_plan plan);
  * No single reference call does this: the reference runs such a batch as one plan per length, through Bluestein for
  * every length that is not a power of two (bluestein.hpp), one job per contiguous range (detail/batch_arms.hpp).
  * Here the transforms are grouped by length into tiles, the tiles balanced over one workgroup per CU, and each length
- * runs its own radix plan (radices 2, 3, 4, 5, 7, 8, 16) chosen by the autotuner. C2C, FP32, unnormalized. */
+ * runs its own radix plan (radices 2-10, 12, 16) chosen by the autotuner. C2C, FP32, unnormalized. */
 typedef struct crdfft_many_plan_s* crdfft_many_plan;
 
 #define CRDFFT_MAX_STAGES 16

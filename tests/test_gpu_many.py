@@ -55,7 +55,7 @@ def test_mixed_lengths_forward_inverse(gpu, po):
     assert 1 <= info["workgroups"] <= 256
 
 
-@pytest.mark.parametrize("n", [12, 60, 64, 1000, 2187, 4096, 6561, 7 * 7 * 7 * 8, 15876, 16384])
+@pytest.mark.parametrize("n", [12, 60, 64, 96, 640, 1000, 2187, 4096, 6561, 7 * 7 * 7 * 8, 15876, 16384])
 def test_every_candidate_is_exact(gpu, po, n):
     from crd_amd import many, workloads
 
