@@ -61,6 +61,28 @@ def time_loop(fn, stream, steps, warmup, Event):
     return e0.elapsed_ms(e1) / steps
 
 
+def fnv1a64(data):
+    """FNV-1a, 64-bit, over the bytes of `data` (the digest the reference's self-check records, used here to show
+    that two runs or builds produced identical output bits)."""
+    h = 0xCBF29CE484222325
+    for byte in bytes(data):
+        h = ((h ^ byte) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return f"{h:016x}"
+
+
+def step_times(fn, stream, reps, Event):
+    """Per-step times (ms) of `reps` steps, each bracketed by its own events."""
+    out = []
+    for _ in range(reps):
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        out.append(e0.elapsed_ms(e1))
+    return out
+
+
 def rel_l2(y, ref):
     return float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
 
@@ -70,6 +92,15 @@ def cpu_baseline(seconds_per_shape, threads):
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     from oracle import pyoracle as po
 
+    # single-core calibration against the reference's published AVX-512 codelet (0.00292 ms per 4096-point f32
+    # transform, level_plans_gen.hpp:83): 256 transforms of n = 4096 on one thread, best of 5
+    x1 = np.random.default_rng(8).standard_normal(2 * 4096 * 256, dtype=np.float32).view(np.complex64).reshape(256, 4096)
+    best1 = float("inf")
+    for _ in range(5):
+        y1 = x1.copy()
+        t0 = time.perf_counter()
+        po.c2c32_inplace(y1, inverse=False, threads=1)
+        best1 = min(best1, time.perf_counter() - t0)
     total_bytes = 0.0
     total_s = 0.0
     parts = []
@@ -103,6 +134,7 @@ def cpu_baseline(seconds_per_shape, threads):
         "kind": "port",
         "sample": "FP32 Stockham/four-step restatement (oracle/), one plan per thread over contiguous ranges: "
                   + "; ".join(parts) + "; scaled to one step of the GPU workload",
+        "single_core_ms_per_n4096": round(1000.0 * best1 / 256, 5),
     }
 
 
@@ -256,6 +288,7 @@ def main():
 
     log(f"timing {args.steps} steps after {args.warmup} warmup")
     ms_step = time_loop(step, stream, args.steps, args.warmup, Event)
+    per_step = sorted(step_times(step, stream, max(20, args.steps), Event))
     step_bytes = sum(16.0 * n * b for n, b in SHAPES)
     value = step_bytes / (ms_step * 1e-3) / 1e9
     result = {
@@ -278,6 +311,8 @@ def main():
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
+        "step_ms_best": round(per_step[0], 4),
+        "step_ms_median": round(per_step[len(per_step) // 2], 4),
     }
 
     # per-shape figures and the live per-kernel roofline
@@ -305,10 +340,12 @@ def main():
             kernels.append(k)
             if best is None or k["total_ms"] > best[0]["total_ms"]:
                 best = (k, n, batch, info)
+        stream.synchronize()
+        head = xout.download((4096,), np.complex64)  # the first 32 KiB of the output of the last execute
         per_shape.append({"n": n, "batch": batch, "ms": round(ms, 4), "gbs": round(gbs, 1),
                           "ffts_per_s": round(batch / (ms * 1e-3), 1), "plan": {k: info[k] for k in (
                               "algorithm", "n1", "n2", "e", "e1", "tile1", "e2", "tile2", "chunk")},
-                          "kernels": kernels})
+                          "kernels": kernels, "output_head_fnv1a64": fnv1a64(head.tobytes())})
     result["per_shape"] = per_shape
     k, n, batch, info = best
     result["roofline"] = {

@@ -17,6 +17,7 @@
 #include "../../include/crdfft.h"
 #include "kernels.hpp"
 #include "planner.hpp"
+#include "scratch.hpp"
 
 namespace
 {
@@ -63,7 +64,8 @@ struct crdfft_nd_plan_s
     std::vector<Axis> axes;
     uint64_t total = 1;
     uint64_t batch = 1;
-    void* work = nullptr;
+    crdfft::DeviceBuffer work; // transpose buffer of the axes that go through transposed lines
+    crdfft::PerStream work_of;
     ~crdfft_nd_plan_s()
     {
         for (Axis& a : axes)
@@ -76,10 +78,6 @@ struct crdfft_nd_plan_s
             {
                 (void)hipFree(a.tw);
             }
-        }
-        if (work != nullptr)
-        {
-            (void)hipFree(work);
         }
     }
 };
@@ -153,7 +151,7 @@ crdfft_status build(crdfft_nd_plan_s* p, const uint64_t* dims, uint32_t ndim, ui
     }
     if (need_work)
     {
-        ND_TRY(hipMalloc(&p->work, p->total * batch * sizeof(crdfft::cfloat)));
+        ND_TRY(p->work.alloc(p->total * batch * sizeof(crdfft::cfloat)));
     }
     ND_TRY(hipStreamSynchronize(nullptr));
     return CRDFFT_OK;
@@ -294,7 +292,11 @@ crdfft_status crdfft_nd_execute(crdfft_nd_plan p, void* data, int32_t direction,
     const bool inverse = direction == CRDFFT_INVERSE;
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto* x = static_cast<crdfft::cfloat*>(data);
-    auto* w = static_cast<crdfft::cfloat*>(p->work);
+    auto* w = static_cast<crdfft::cfloat*>(p->work_of.get(p->work, s));
+    if (w == nullptr && p->work.bytes != 0)
+    {
+        return CRDFFT_OUT_OF_MEMORY;
+    }
     double scale = 1.0;
     for (const Axis& a : p->axes)
     {

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -15,11 +16,15 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/crdfft.h"
+#include "scratch.hpp"
 #include "../../include/crdfft_runtime.h"
 #include "kernels.hpp"
 #include "mfma.hpp"
 #include "planner.hpp"
 #include "real.hpp"
+
+using crdfft::DeviceBuffer;
+using crdfft::PerStream;
 
 namespace
 {
@@ -55,29 +60,6 @@ crdfft_status hip_status(hipError_t e)
 using crdfft::cfloat;
 
 // A device buffer the plan owns.
-struct DeviceBuffer
-{
-    void* ptr = nullptr;
-    uint64_t bytes = 0;
-    hipError_t alloc(uint64_t b)
-    {
-        bytes = b;
-        if (b == 0)
-        {
-            return hipSuccess;
-        }
-        return hipMalloc(&ptr, b);
-    }
-    ~DeviceBuffer()
-    {
-        if (ptr != nullptr)
-        {
-            (void)hipFree(ptr);
-        }
-    }
-    template <typename T> T* as() const { return static_cast<T*>(ptr); }
-};
-
 } // namespace
 
 struct crdfft_plan_s
@@ -103,6 +85,7 @@ struct crdfft_plan_s
     crdfft_plan inner = nullptr;
     DeviceBuffer chirp, bhat, bwork;
     uint64_t blue_chunk = 0;
+    PerStream workspace_of, bwork_of; // per-stream copies of workspace / bwork
     std::mutex ref_mutex;
     ~crdfft_plan_s()
     {
@@ -302,11 +285,16 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
         }
         case CRDFFT_ALGO_FOUR_STEP:
         {
+            cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
+            if (ws == nullptr)
+            {
+                return CRDFFT_OUT_OF_MEMORY;
+            }
             for (uint64_t c0 = 0; c0 < count; c0 += sh.chunk)
             {
                 const uint64_t cnt = std::min(sh.chunk, count - c0);
                 crdfft::ColArgs ca{in + c0 * idist,
-                                   p->workspace.as<cfloat>(),
+                                   ws,
                                    p->tw1.as<cfloat>(),
                                    p->thi.as<cfloat>(),
                                    p->tlo.as<cfloat>(),
@@ -328,7 +316,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                     CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
                 }
                 CRDFFT_TRY(prof.after(0, cn * cnt));
-                crdfft::RowTrArgs ra{p->workspace.as<cfloat>(), out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
+                crdfft::RowTrArgs ra{ws, out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
                                      cn, odist, scale, sh.hints & crdfft::kHintNtStore};
                 CRDFFT_TRY(prof.before());
                 CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
@@ -338,9 +326,14 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
         }
         case CRDFFT_ALGO_FUSED:
         {
+            cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
+            if (ws == nullptr)
+            {
+                return CRDFFT_OUT_OF_MEMORY;
+            }
             crdfft::FusedArgs fa{in,
                                  out,
-                                 p->workspace.as<cfloat>(),
+                                 ws,
                                  p->tw1.as<cfloat>(),
                                  p->tw2.as<cfloat>(),
                                  p->tfull.as<cfloat>(),
@@ -380,15 +373,20 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         const cfloat* x = static_cast<const cfloat*>(in);
         cfloat* y = static_cast<cfloat*>(out);
         const float* tab = inverse ? p->split_inv.as<float>() : p->split_fwd.as<float>();
+        cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
+        if (ws == nullptr)
+        {
+            return CRDFFT_OUT_OF_MEMORY;
+        }
         for (uint64_t c0 = 0; c0 < sh.batch; c0 += sh.chunk)
         {
             const uint64_t cnt = std::min(sh.chunk, sh.batch - c0);
-            crdfft::SplitArgs a{x + c0 * p->idist, p->workspace.as<cfloat>(), tab, p->tw256.as<cfloat>(),
+            crdfft::SplitArgs a{x + c0 * p->idist, ws, tab, p->tw256.as<cfloat>(),
                                 p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, p->idist, sh.complex_n};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(0, sh.precision, inverse, a, s));
             CRDFFT_TRY(prof.after(0, sh.complex_n * cnt));
-            crdfft::SplitArgs b{p->workspace.as<cfloat>(), y + c0 * p->odist, tab, p->tw256.as<cfloat>(),
+            crdfft::SplitArgs b{ws, y + c0 * p->odist, tab, p->tw256.as<cfloat>(),
                                 p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, sh.complex_n, p->odist};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(1, sh.precision, inverse, b, s));
@@ -401,7 +399,11 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         const uint64_t n = sh.n, m = sh.n1;
         const cfloat* x = static_cast<const cfloat*>(in);
         cfloat* y = static_cast<cfloat*>(out);
-        cfloat* w = p->bwork.as<cfloat>();
+        cfloat* w = static_cast<cfloat*>(p->bwork_of.get(p->bwork, s));
+        if (w == nullptr)
+        {
+            return CRDFFT_OUT_OF_MEMORY;
+        }
         for (uint64_t c0 = 0; c0 < sh.batch; c0 += p->blue_chunk)
         {
             const uint64_t cnt = std::min(p->blue_chunk, sh.batch - c0);

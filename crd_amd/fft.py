@@ -243,11 +243,12 @@ class BluesteinPlan(FftPlan):
 class NdFftPlan:
     """Row-major N-D C2C of `batch` volumes in place (nd_fft.hpp:28-196): inverse scaled by 1/prod(dims)."""
 
-    def __init__(self, dims, batch=1):
+    def __init__(self, dims, batch=1, transpose_all=False):
         self.dims = [int(d) for d in dims]
         arr = (ctypes.c_uint64 * len(self.dims))(*self.dims)
         self.handle = ctypes.c_void_p()
-        check(N.lib().crdfft_nd_plan_create(ctypes.byref(self.handle), arr, len(self.dims), int(batch), 0),
+        flags = N.ND_TRANSPOSE_ALL if transpose_all else 0
+        check(N.lib().crdfft_nd_plan_create(ctypes.byref(self.handle), arr, len(self.dims), int(batch), flags),
               f"crdfft_nd_plan_create({self.dims})")
 
     def execute(self, data, direction=N.FORWARD, stream=None):

@@ -25,9 +25,11 @@
  * C2C is unnormalized in both directions; C2R (irfft) scales by 1/n. Complex data is interleaved FP32 {re, im}.
  * Errors: the reference asserts on contract violations and aborts on allocation failure (fft.hpp:52, :237); here
  * every entry point returns a status and never aborts the host process. NaN and Inf propagate as they do on the CPU.
- * Threading: a plan is immutable after creation except for its workspace; calls on one plan from two streams at
- * once race on the workspace of four-step plans (one plan per stream, as the reference keeps one plan per thread,
- * fft.hpp:17-18).
+ * Threading: a plan is immutable after creation. Its scratch (four-step intermediate, Bluestein and N-D work
+ * buffers) is per stream: the buffer allocated with the plan serves the first stream that executes it, and the first
+ * execute on any other stream allocates that stream's own (up to 64 streams; beyond, CRDFFT_OUT_OF_MEMORY), so host
+ * threads may share one plan on distinct streams (the reference keeps one plan per thread, fft.hpp:17-18). Calls on
+ * one stream are ordered by the stream. The first execute on a new stream must not be inside a graph capture.
  */
 #ifndef CRDFFT_H
 #define CRDFFT_H

@@ -311,3 +311,54 @@ def test_nd_plan(gpu, po, dims):
         xd = x.astype(np.complex128)
         ref = np.fft.ifftn(xd, axes=axes) if inverse else np.fft.fftn(xd, axes=axes)
         assert po.rel_l2(y, ref) <= 3 * bound(max(total, 2)), f"dims={dims} inverse={inverse}"
+
+
+@pytest.mark.parametrize("kind", ["four_step", "fused", "bluestein", "nd"])
+def test_one_plan_on_several_streams(gpu, po, kind):
+    """One plan executed on three streams at once, interleaved without synchronisation: each stream has its own
+    scratch, so every result equals the plan's result on one stream (the reference's plans are per thread,
+    fft.hpp:17-18; here one plan may be shared)."""
+    streams = [gpu.Stream() for _ in range(3)]
+    if kind == "nd":
+        dims, batch = (16, 64, 64), 2
+        n = int(np.prod(dims))
+        plan = gpu.NdFftPlan(dims, batch, transpose_all=True)
+    else:
+        n, batch = {"four_step": (1 << 16, 24), "fused": (1 << 16, 24), "bluestein": (3 * 5 * 7 * 11, 40)}[kind]
+        if kind == "bluestein":
+            plan = gpu.BluesteinPlan(n, batch)
+        else:
+            plan = gpu.FftPlan(n, batch, tuning={"algorithm": 5} if kind == "fused" else {"algorithm": 2, "chunk": 8})
+    xs = [po.lcg_complex(n * batch, 1000 + i).reshape(batch, n) for i in range(3)]
+    bufs = [gpu.DeviceBuffer.from_array(x) for x in xs]
+    outs = [gpu.DeviceBuffer(x.nbytes) for x in xs]
+
+    def run(i, s):
+        if kind == "nd":
+            plan.execute(outs[i], gpu.FORWARD, s)
+        else:
+            plan.execute(bufs[i], outs[i], gpu.FORWARD, s)
+
+    # reference: one stream, one at a time
+    want = []
+    for i in range(3):
+        if kind == "nd":
+            outs[i].copy_from(bufs[i])
+            gpu.synchronize()  # the copy runs on the null stream, which the library's streams do not wait for
+        run(i, streams[0])
+        streams[0].synchronize()
+        want.append(outs[i].download(xs[i].shape, np.complex64))
+    # three streams interleaved, several rounds
+    for _ in range(4):
+        if kind == "nd":
+            gpu.synchronize()
+            for i in range(3):
+                outs[i].copy_from(bufs[i])
+            gpu.synchronize()
+        for i in range(3):
+            run(i, streams[i])
+        for s in streams:
+            s.synchronize()
+        for i in range(3):
+            got = outs[i].download(xs[i].shape, np.complex64)
+            assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), f"{kind}: stream {i} differs"
