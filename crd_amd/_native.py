@@ -19,6 +19,8 @@ FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
 ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED = 0, 1, 2, 3, 4, 5
 FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
 ND_TRANSPOSE_ALL = 1
+WINDOW_BOXCAR, WINDOW_HANN = 0, 1
+SPECTRAL_DETREND = 1
 
 
 class CrdfftError(RuntimeError):
@@ -175,6 +177,12 @@ _SIGS = {
     "crdfft_many_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_many_graph_capture": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_many_graph_launch": (_i32, [_vp, _vp]),
+    "crdfft_spectral_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _i32, _u32]),
+    "crdfft_spectral_plan_destroy": (_i32, [_vp]),
+    "crdfft_spectral_frames": (_u64, [_vp, _u64]),
+    "crdfft_stft": (_i32, [_vp, _vp, _u64, _vp, _vp]),
+    "crdfft_spectrogram": (_i32, [_vp, _vp, _u64, ctypes.c_double, _vp, _vp]),
+    "crdfft_welch": (_i32, [_vp, _vp, _u64, ctypes.c_double, _vp, _vp]),
     "crdfft_many_autotune": (_i32, [ctypes.POINTER(_u64), _u64, _u32, ctypes.POINTER(RadixPlan), _u64,
                                     ctypes.POINTER(_u64), ctypes.POINTER(TuneRecord), _u64, ctypes.POINTER(_u64)]),
     # single-length autotuner, records and features

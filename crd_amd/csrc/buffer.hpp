@@ -28,6 +28,13 @@ template <int AUX> __device__ __forceinline__ cf bld(__amdgpu_buffer_rsrc_t r, u
     return cf{f.x, f.y};
 }
 
+template <int AUX> __device__ __forceinline__ float bld32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    const auto u = __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(voff), static_cast<int>(soff), AUX);
+    static_assert(sizeof(u) == 4, "32-bit buffer access");
+    return __builtin_bit_cast(float, u);
+}
+
 template <int AUX> __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, cf v)
 {
     f2v f;

@@ -1,0 +1,281 @@
+// The spectral front end over the real FFT, fused (the reference's hesap-dsp spectral.hpp:43-126, :140-172,
+// :334-386: Welch PSD, spectrogram, periodogram and STFT, which there run frame by frame as detrend + window loops,
+// a RealFftPlan rfft and a |X|^2 loop).
+//
+// k_spec<H, E, MODE>: one real frame of N = 2H samples per group of T = H/E threads (several frames per workgroup
+// below 256 threads). Each thread loads the sample pairs of its complex slots j = i + T m straight from the signal
+// (frame s starts at s * hop, any hop), subtracts the frame mean (a workgroup tree reduction, fixed order) when
+// asked, multiplies by the window, runs the H-point Stockham core on z[j] = x[2j] + i x[2j+1], and does the
+// Hermitian recombine X[k] = E[k] + W_N^k O[k], E = (Z[k] + conj Z[H-k])/2, O = (Z[k] - conj Z[H-k])/(2i), with
+// Z[H-k] read back through LDS; X[H] = Re Z[0] - Im Z[0] comes from the k = 0 slot. The epilogue then writes
+//   MODE 0  the complex bins (STFT),
+//   MODE 1  the density periodogram scale |X|^2, doubled except DC and Nyquist (spectrogram rows),
+//   MODE 2  nothing per frame: lane group g sums the periodograms of frames g, g + G, g + 2G, ... in registers and
+//           writes one partial row; k_spec_sum adds the G partial rows in order (Welch, bit-identical run to run).
+//           Frames of 32768 samples run MODE 1 in chunks of rows instead, each chunk added in order.
+// HBM traffic: the signal once per frame that covers it (twice at 50% overlap) and the output; no framed copy of
+// the signal, no spectrum round trip.
+#include "buffer.hpp"
+#include "fft_core.hpp"
+#include "kernels.hpp"
+#include "spectral.hpp"
+
+#include <algorithm>
+
+namespace crdfft
+{
+using namespace dev;
+
+namespace
+{
+
+constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+template <int H, int E> struct SpecCfg
+{
+    static constexpr int T = H / E;
+    static constexpr int M = T >= 256 ? 1 : 256 / T; // frames per workgroup
+    static constexpr int threads = T * M;
+    static constexpr int LF = lds_floats(H);
+    static constexpr int RS = round_up(2 * LF, 64) + (T < 64 ? T : 0); // per-frame region (re then im)
+    static constexpr int lds_floats_total = RS * M + threads;         // + the mean reduction
+    static constexpr int lds_bytes = lds_floats_total * 4;
+};
+
+template <int H, int E, int MODE>
+__global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
+{
+    using C = SpecCfg<H, E>;
+    constexpr int T = C::T;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int local = tid / T;
+    const int i = tid - local * T;
+    const LdsView<1> view{lds + local * C::RS, C::LF};
+    float* red = lds + C::RS * C::M;
+    const uint64_t group = static_cast<uint64_t>(blockIdx.x) * C::M + local;
+    const uint64_t groups = static_cast<uint64_t>(gridDim.x) * C::M;
+    // Welch accumulators (MODE 2 exists for H <= 8192: the largest core needs every register of its 1024 threads)
+    static_assert(MODE != 2 || H <= kSpecWelchMaxH, "Welch lane groups up to H = 8192");
+    float acc[E + 1];
+#pragma unroll
+    for (int m = 0; m <= E; ++m)
+    {
+        acc[m] = 0.0f;
+    }
+    // MODE 0/1: one frame per lane group; MODE 2: frames group, group + groups, ... (uniform trip count per
+    // workgroup: every lane group runs the same number of iterations, idle ones on a masked frame)
+    const uint64_t iters = MODE == 2 ? (a.frames + groups - 1) / groups : 1;
+#pragma unroll 1
+    for (uint64_t it = 0; it < iters; ++it)
+    {
+        const uint64_t f = group + it * groups;
+        const bool active = f < a.frames;
+        // the tables are the same for every frame; an empty asm hides that, so the compiler reloads them per frame
+        // (L1/L2 hits) instead of holding every butterfly's twiddles in registers across the frame loop
+        const cf* tw = reinterpret_cast<const cf*>(a.tw);
+        const cf* wk = reinterpret_cast<const cf*>(a.wtab);
+        const float* win = a.window;
+        asm volatile("" : "+s"(tw), "+s"(wk), "+s"(win));
+        // the frame through a buffer resource: an inactive frame gets no records, so its loads read zeros and touch
+        // no memory
+        const __amdgpu_buffer_rsrc_t xr = make_rsrc(a.x + (active ? f * a.hop : 0), active ? 8u * H : 0u);
+        cf v[E];
+        float part = 0.0f;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            const uint32_t j = static_cast<uint32_t>(i + T * m);
+            v[m] = cf{bld32<0>(xr, 8u * j, 0), bld32<0>(xr, 8u * j + 4u, 0)};
+            part += v[m].x + v[m].y;
+        }
+        if (a.detrend != 0)
+        {
+            // the frame mean: tree sum over the frame's T threads, fixed order
+            red[tid] = part;
+            __syncthreads();
+#pragma unroll
+            for (int s = T / 2; s > 0; s >>= 1)
+            {
+                if (i < s)
+                {
+                    red[tid] += red[tid + s];
+                }
+                __syncthreads();
+            }
+            const float mean = red[local * T] / static_cast<float>(2 * H);
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                v[m].x -= mean;
+                v[m].y -= mean;
+            }
+        }
+        if (a.window != nullptr)
+        {
+            const __amdgpu_buffer_rsrc_t wr = make_rsrc(win, 8u * H);
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                const uint32_t j = static_cast<uint32_t>(i + T * m);
+                v[m].x *= bld32<0>(wr, 8u * j, 0);
+                v[m].y *= bld32<0>(wr, 8u * j + 4u, 0);
+            }
+        }
+        fft_core<H, E, false, 1>(v, i, view, tw);
+        // Z through LDS for the recombine (fft_core's last LDS read is behind a barrier; with H == E it used none)
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            view.put(i + T * m, v[m]);
+        }
+        __syncthreads();
+        // recombine and epilogue per slot (nothing but the accumulators outlives the frame)
+        const uint64_t row = f * static_cast<uint64_t>(H + 1);
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            const int k = i + T * m;
+            const cf zk = v[m];
+            const cf zh = view.get(k == 0 ? 0 : H - k);
+            const cf e = {(zk.x + zh.x) * 0.5f, (zk.y - zh.y) * 0.5f};
+            const cf o = {(zk.y + zh.y) * 0.5f, (zh.x - zk.x) * 0.5f};
+            const cf xk = cadd(e, cmul(o, wk[k]));
+            // k == 0 also yields the Nyquist bin: W_N^H = -1, X[H] = Re Z0 - Im Z0
+            const cf xn = cadd(e, cmul(o, wk[H]));
+            if constexpr (MODE == 0)
+            {
+                if (active)
+                {
+                    cf* out = reinterpret_cast<cf*>(a.out) + row;
+                    out[k] = xk;
+                    if (k == 0)
+                    {
+                        out[H] = xn;
+                    }
+                }
+            }
+            else
+            {
+                const float p = a.scale * (xk.x * xk.x + xk.y * xk.y) * (k == 0 ? 1.0f : 2.0f);
+                const float pn = a.scale * (xn.x * xn.x + xn.y * xn.y);
+                if constexpr (MODE == 1)
+                {
+                    if (active)
+                    {
+                        float* out = reinterpret_cast<float*>(a.out) + row;
+                        out[k] = p;
+                        if (k == 0)
+                        {
+                            out[H] = pn;
+                        }
+                    }
+                }
+                else
+                {
+                    acc[m] += active ? p : 0.0f;
+                    if (k == 0)
+                    {
+                        acc[E] += active ? pn : 0.0f;
+                    }
+                }
+            }
+        }
+        __syncthreads(); // LDS free for the next frame
+    }
+    if constexpr (MODE == 2)
+    {
+        // one partial row per lane group
+        float* prow = reinterpret_cast<float*>(a.out) + group * static_cast<uint64_t>(H + 1);
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            prow[i + T * m] = acc[m];
+        }
+        if (i == 0)
+        {
+            prow[H] = acc[E];
+        }
+    }
+}
+
+// acc[k] = ((first ? 0 : acc[k]) + sum over r < rows of rows[r][k], in order) * scale.
+__global__ void k_spec_sum(const float* rows, float* acc, uint64_t nfreq, uint64_t nrows, int first, float scale)
+{
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nfreq)
+    {
+        return;
+    }
+    float s = first != 0 ? 0.0f : acc[k];
+    for (uint64_t r = 0; r < nrows; ++r)
+    {
+        s += rows[r * nfreq + k];
+    }
+    acc[k] = s * scale;
+}
+
+#define CRDFFT_SPEC_LIST(X)                                                                                            \
+    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(512, 16) X(1024, 16) X(2048, 16)   \
+        X(4096, 16) X(8192, 16) X(16384, 16)
+
+template <typename K> hipError_t set_lds(K kernel, unsigned bytes)
+{
+    if (bytes > 64 * 1024)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(bytes));
+    }
+    return hipSuccess;
+}
+
+} // namespace
+
+bool spec_shape(uint64_t h, int* e, unsigned* frames_per_group)
+{
+#define X(H, E)                                                                                                        \
+    if (h == H)                                                                                                        \
+    {                                                                                                                  \
+        *e = E;                                                                                                        \
+        *frames_per_group = SpecCfg<H, E>::M;                                                                          \
+        return true;                                                                                                   \
+    }
+    CRDFFT_SPEC_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_spec(uint64_t h, int mode, const SpecArgs& a, uint64_t groups, hipStream_t s)
+{
+#define X(H, E)                                                                                                        \
+    if (h == H)                                                                                                        \
+    {                                                                                                                  \
+        using C = SpecCfg<H, E>;                                                                                       \
+        if (mode == 2 && H > kSpecWelchMaxH)                                                                           \
+            return hipErrorInvalidValue;                                                                               \
+        auto kern = mode == 0 ? k_spec<H, E, 0>                                                                        \
+                              : (mode == 1 ? k_spec<H, E, 1> : k_spec<H, E, (H <= kSpecWelchMaxH ? 2 : 1)>);        \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        if (groups == 0 || groups > 0xFFFFFFFFull)                                                                     \
+            return hipErrorInvalidValue;                                                                               \
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, a);        \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_SPEC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_spec_sum(const float* rows, float* acc, uint64_t nfreq, uint64_t nrows, bool first, float scale,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spec_sum, dim3(static_cast<unsigned>((nfreq + 255) / 256)), dim3(256), 0, s, rows, acc, nfreq,
+                       nrows, first ? 1 : 0, scale);
+    return hipGetLastError();
+}
+
+} // namespace crdfft

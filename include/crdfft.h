@@ -306,6 +306,36 @@ CRDFFT_API crdfft_status crdfft_mlp_train(crdfft_mlp mlp, const float* x, const 
                                           uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
                                           float* final_loss);
 
+/* ---- the spectral front end over the real FFT (hesap-dsp spectral.hpp) ----
+ * A plan of real frames of nperseg samples (a power of two, 4..32768) at hop = nperseg - noverlap: frame s is
+ * x[s*hop, s*hop + nperseg), and a signal of len samples has (len - noverlap)/hop frames (0 if len < nperseg), as
+ * segment_periodograms_arm / stft_arm count them (spectral.hpp:53-56, :342-345). Every call is one fused launch
+ * (plus a small reduction for Welch) on device memory: frame load, optional mean removal, window, rfft, epilogue. */
+typedef struct crdfft_spectral_plan_s* crdfft_spectral_plan;
+typedef enum crdfft_window
+{
+    CRDFFT_WINDOW_BOXCAR = 0,
+    CRDFFT_WINDOW_HANN = 1 /* periodic hann (windows.hpp:120-124, sym = false), FP64-built */
+} crdfft_window;
+#define CRDFFT_SPECTRAL_DETREND 1u /* subtract each frame's mean (detrend='constant') */
+CRDFFT_API crdfft_status crdfft_spectral_plan_create(crdfft_spectral_plan* plan, uint64_t nperseg, uint64_t noverlap,
+                                                    int32_t window, uint32_t flags);
+CRDFFT_API crdfft_status crdfft_spectral_plan_destroy(crdfft_spectral_plan plan);
+CRDFFT_API uint64_t crdfft_spectral_frames(crdfft_spectral_plan plan, uint64_t len);
+/* stft (spectral.hpp:334-386, window hann, no detrend): out = frames x (nperseg/2+1) complex, unscaled rfft of each
+ * windowed frame. */
+CRDFFT_API crdfft_status crdfft_stft(crdfft_spectral_plan plan, const float* x, uint64_t len, void* out, void* stream);
+/* spectrogram / segment periodograms (spectral.hpp:43-126, 175-184): out = frames x (nperseg/2+1) floats, the
+ * density |X|^2 / (fs * sum w^2), doubled except DC and Nyquist. periodogram (spectral.hpp:186-222) is one boxcar
+ * frame of the whole signal (nperseg = len, noverlap 0). */
+CRDFFT_API crdfft_status crdfft_spectrogram(crdfft_spectral_plan plan, const float* x, uint64_t len, double fs,
+                                            float* out, void* stream);
+/* welch_psd (spectral.hpp:140-172): psd = nperseg/2+1 floats, the frames' density periodograms averaged. The sum runs
+ * in a fixed order (lane-group partial rows, then rows in order), so the result is bit-identical run to run. Uses
+ * the plan's scratch for the partial rows (per stream). All zeros when there is no frame. */
+CRDFFT_API crdfft_status crdfft_welch(crdfft_spectral_plan plan, const float* x, uint64_t len, double fs, float* psd,
+                                      void* stream);
+
 CRDFFT_API const char* crdfft_status_string(crdfft_status status);
 CRDFFT_API int32_t crdfft_last_hip_error(void);
 CRDFFT_API const char* crdfft_version(void);

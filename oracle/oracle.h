@@ -23,6 +23,9 @@
  *                    and a planted wrong twiddle sign that the tests must catch.
  *   oracle_*32_batch bit-identity across thread counts (batch_arms.hpp:11-14) and FP64 truth.
  *   oracle_bluestein32   FP64 truth for primes and 7-smooth n.
+ *   oracle_welch64 / oracle_periodograms64   the reference's scipy vectors (spectral_refs.inc): Welch, spectrogram,
+ *                    periodogram; oracle_stft64 numpy rfft of explicitly windowed frames; oracle_hann64 the closed form
+ *                    and scipy's hann.
  */
 #pragma once
 #include <stddef.h>
@@ -74,6 +77,20 @@ void oracle_irfft32_batch(const float* in, float* out, size_t n, size_t batch, i
 void oracle_bluestein32(float* data, size_t n, int inverse);
 /* Hardware threads the batch functions use for threads <= 0. */
 int oracle_hardware_threads(void);
+
+/* ---- FP64 spectral front end over the real FFT (hesap-dsp spectral.hpp / windows.hpp; spectral.cpp) ----
+ * Frames s = x[s*hop, s*hop + nperseg), hop = nperseg - noverlap, count (len - noverlap)/hop (0 if len < nperseg). */
+/* hann window of m points, symmetric (sym != 0) or periodic (windows.hpp:63-124). */
+void oracle_hann64(double* out, size_t m, int sym);
+/* Density periodogram of every frame: window 1 = hann periodic, 0 = boxcar; detrend 1 subtracts each frame's mean;
+ * out nseg x (nperseg/2 + 1); returns nseg (segment_periodograms_arm, spectral.hpp:43-126; periodogram = one boxcar
+ * frame of the whole signal, :186-222). */
+size_t oracle_periodograms64(const double* x, size_t len, double fs, size_t nperseg, size_t noverlap, int window,
+                             int detrend, double* out);
+/* Welch PSD: the hann, detrend-constant periodograms averaged in frame order (spectral.hpp:140-172). */
+size_t oracle_welch64(const double* x, size_t len, double fs, size_t nperseg, size_t noverlap, double* psd);
+/* STFT: complex rfft of every hann-windowed frame, no detrend, no scaling (spectral.hpp:334-386). */
+size_t oracle_stft64(const double* x, size_t len, size_t nperseg, size_t noverlap, double* out);
 
 /* ---- FP64 restatement of the plan cost model (mlp.cpp): 3-layer ReLU perceptron, flat parameters
  * W1[h][in], b1[h], W2[h][h], b2[h], w3[h], b3; pinned by central finite differences. ---- */

@@ -48,6 +48,10 @@ def lib():
             "oracle_irfft32_batch": (None, [vp, vp, sz, sz, i]),
             "oracle_bluestein32": (None, [vp, sz, i]),
             "oracle_hardware_threads": (i, []),
+            "oracle_hann64": (None, [vp, sz, i]),
+            "oracle_periodograms64": (sz, [vp, sz, ctypes.c_double, sz, sz, i, i, vp]),
+            "oracle_welch64": (sz, [vp, sz, ctypes.c_double, sz, sz, vp]),
+            "oracle_stft64": (sz, [vp, sz, sz, sz, vp]),
             "oracle_mlp_forward": (None, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp]),
             "oracle_mlp_loss_grad": (ctypes.c_double, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp]),
         }
@@ -113,6 +117,43 @@ def rfft64(x):
     for r in range(rows.shape[0]):
         lib().oracle_rfft64(_p(rows[r]), _p(out[r]), n)
     return out.reshape(x.shape[:-1] + (n // 2 + 1,))
+
+
+def frames(length, nperseg, noverlap):
+    hop = nperseg - noverlap
+    return 0 if length < nperseg else (length - noverlap) // hop
+
+
+def hann64(m, sym=False):
+    out = np.empty(m, dtype=np.float64)
+    lib().oracle_hann64(_p(out), m, 1 if sym else 0)
+    return out
+
+
+def periodograms64(x, fs, nperseg, noverlap, window="hann", detrend=True):
+    """Density periodogram rows of every frame (spectrogram), FP64."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nseg = frames(len(x), nperseg, noverlap)
+    out = np.empty((max(nseg, 1), nperseg // 2 + 1), dtype=np.float64)
+    got = lib().oracle_periodograms64(_p(x), len(x), float(fs), nperseg, noverlap, 1 if window == "hann" else 0,
+                                      1 if detrend else 0, _p(out))
+    return out[:got]
+
+
+def welch64(x, fs, nperseg, noverlap=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    noverlap = nperseg // 2 if noverlap is None else noverlap
+    out = np.empty(nperseg // 2 + 1, dtype=np.float64)
+    lib().oracle_welch64(_p(x), len(x), float(fs), nperseg, noverlap, _p(out))
+    return out
+
+
+def stft64(x, nperseg, noverlap):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nf = frames(len(x), nperseg, noverlap)
+    out = np.empty((max(nf, 1), nperseg // 2 + 1), dtype=np.complex128)
+    got = lib().oracle_stft64(_p(x), len(x), nperseg, noverlap, _p(out))
+    return out[:got]
 
 
 def irfft64(X, n):

@@ -1,5 +1,7 @@
 """Pins the CPU oracle (oracle/): FP64 truth against numpy's pocketfft fixtures, closed forms and invariants; the FP32
 restatement against FP64 at the reference's tolerance classes; a planted mistake that must be caught."""
+import os
+
 import numpy as np
 import pytest
 
@@ -178,3 +180,60 @@ def test_mlp_gradient_matches_finite_differences(po):
         num[i] = (np.mean((po.mlp_forward(pp, inputs, hidden, x) - t) ** 2)
                   - np.mean((po.mlp_forward(pm, inputs, hidden, x) - t) ** 2)) / (2 * h)
     assert np.max(np.abs(num - g)) <= 1e-6 * max(1.0, np.max(np.abs(g)))
+
+
+# ---- the spectral front end (hesap-dsp spectral.hpp), pinned by the reference's own scipy vectors ----
+
+SPECTRAL_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "spectral_golden.npz")
+
+
+@pytest.fixture(scope="module")
+def spectral_golden():
+    with np.load(SPECTRAL_GOLDEN, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_welch_oracle_matches_reference_vectors(po, spectral_golden):
+    """welch_psd(x, fs=100, nperseg=64) against the reference fixture at its own gate, 1e-11 relative
+    (test_spectral.cpp:33-41)."""
+    g = spectral_golden
+    psd = po.welch64(g["x"], 100.0, 64)
+    assert psd.shape == g["welch_pxx"].shape
+    assert np.all(np.abs(psd - g["welch_pxx"]) <= 1e-11 * np.abs(g["welch_pxx"]))
+
+
+def test_spectrogram_oracle_matches_reference_vectors(po, spectral_golden):
+    """spectrogram(x, 100, 64, 32): 15 x 33 rows, 1e-9 relative or 1e-11 absolute (test_spectral.cpp:79-88)."""
+    g = spectral_golden
+    s = po.periodograms64(g["x"], 100.0, 64, 32, window="hann", detrend=True)
+    assert s.shape == g["spec"].shape
+    d = np.abs(s - g["spec"])
+    assert np.all((d <= 1e-9 * np.abs(g["spec"])) | (d <= 1e-11))
+
+
+def test_periodogram_oracle_matches_reference_vectors(po, spectral_golden):
+    """periodogram(x, 100): one boxcar frame of all 512 samples (test_spectral.cpp:226-231)."""
+    g = spectral_golden
+    p = po.periodograms64(g["x"], 100.0, len(g["x"]), 0, window="boxcar", detrend=True)[0]
+    d = np.abs(p - g["periodogram"])
+    assert np.all((d <= 1e-9 * np.abs(g["periodogram"])) | (d <= 1e-12))
+
+
+def test_hann_and_stft_oracle(po):
+    """The periodic hann of windows.hpp is 0.5 - 0.5 cos(2 pi i / m); the STFT frames are numpy's rfft of the
+    windowed frames (numpy's pocketfft, FP64); frame count (len - noverlap) / hop."""
+    for m in (4, 64, 1000):
+        i = np.arange(m)
+        assert np.allclose(po.hann64(m), 0.5 - 0.5 * np.cos(2 * np.pi * i / m), atol=1e-15, rtol=0)
+        assert np.allclose(po.hann64(m, sym=True), 0.5 - 0.5 * np.cos(2 * np.pi * i / (m - 1)), atol=1e-15, rtol=0)
+    x = np.random.default_rng(3).standard_normal(1000)
+    for nperseg, noverlap in ((64, 32), (128, 0), (256, 201)):
+        z = po.stft64(x, nperseg, noverlap)
+        hop = nperseg - noverlap
+        assert z.shape[0] == (len(x) - noverlap) // hop
+        w = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(nperseg) / nperseg)
+        ref = np.array([np.fft.rfft(x[f * hop:f * hop + nperseg] * w) for f in range(z.shape[0])])
+        assert np.max(np.abs(z - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # a signal shorter than one frame has no frames, and Welch of it is all zeros
+    assert po.stft64(x[:63], 64, 32).shape[0] == 0
+    assert np.all(po.welch64(x[:63], 1.0, 64) == 0)
