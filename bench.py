@@ -205,6 +205,42 @@ def bench_mixed(stream, Event):
     return out
 
 
+def bench_spectral(stream, Event):
+    """The fused spectral front end on a 2^26-sample FP32 signal (256 MiB), frames of 4096 at 50% overlap (32767
+    frames): Welch PSD, spectrogram rows and STFT, beside a plain R2C plan over the same frames already cut and
+    windowed (the FFT alone, on a framed copy the fused path never writes)."""
+    import crd_amd
+    from crd_amd import DeviceBuffer, FftPlan, spectral
+
+    length, nperseg, noverlap = 1 << 26, 4096, 2048
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(length, dtype=np.float32)
+    dx = DeviceBuffer.from_array(x)
+    wplan = spectral.SpectralPlan(nperseg, noverlap, "hann", detrend=True)
+    splan = spectral.SpectralPlan(nperseg, noverlap, "hann", detrend=False)
+    frames = wplan.frames(length)
+    nfreq = nperseg // 2 + 1
+    psd = DeviceBuffer(nfreq * 4)
+    rows = DeviceBuffer(frames * nfreq * 8)
+    read_bytes = 4.0 * frames * nperseg  # each frame's samples once (the overlap reads the signal twice)
+    out = {"signal_samples": length, "nperseg": nperseg, "noverlap": noverlap, "frames": frames}
+    ms = time_loop(lambda: wplan.welch(dx, length, 1.0, psd, stream), stream, 10, 2, Event)
+    out["welch"] = {"ms": round(ms, 4), "frames_per_s": round(frames / (ms * 1e-3), 1),
+                    "gbs_frame_reads": round(read_bytes / (ms * 1e-3) / 1e9, 1)}
+    ms = time_loop(lambda: wplan.spectrogram(dx, length, 1.0, rows, stream), stream, 10, 2, Event)
+    out["spectrogram"] = {"ms": round(ms, 4),
+                          "gbs": round((read_bytes + 4.0 * frames * nfreq) / (ms * 1e-3) / 1e9, 1)}
+    ms = time_loop(lambda: splan.stft(dx, length, rows, stream), stream, 10, 2, Event)
+    out["stft"] = {"ms": round(ms, 4), "gbs": round((read_bytes + 8.0 * frames * nfreq) / (ms * 1e-3) / 1e9, 1)}
+    # the FFT alone on a framed, windowed copy (what a frame-by-frame port would run after its own framing pass)
+    framed = DeviceBuffer(frames * nperseg * 4)
+    r2c = FftPlan(nperseg, frames, "r2c", self_check=False)
+    ms = time_loop(lambda: r2c.execute(framed, rows, crd_amd.FORWARD, stream), stream, 10, 2, Event)
+    out["r2c_of_framed_copy_ms"] = round(ms, 4)
+    out["welch_vs_r2c_alone"] = round(out["r2c_of_framed_copy_ms"] / out["welch"]["ms"], 2)
+    return out
+
+
 def bench_split(stream, Event):
     """n = 2^16, batch 2048, |x| log-uniform in [1e-4, 1e4] with uniform phase: plain FP32 against the split-BF16
     and split-FP16 MFMA forms, time and rel-L2 against FP64 (numpy, float64) on a sample."""
@@ -443,6 +479,7 @@ def main():
                                      "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
         extras["mixed_size"] = bench_mixed(stream, Event)
         extras["split_precision"] = bench_split(stream, Event)
+        extras["spectral"] = bench_spectral(stream, Event)
         result["extras"] = extras
 
     if not args.no_cpu:

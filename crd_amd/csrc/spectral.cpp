@@ -1,6 +1,7 @@
 // The spectral front end's C-ABI (crdfft_spectral_*, crdfft_stft, crdfft_spectrogram, crdfft_welch): the GPU form of
 // the reference's hesap-dsp stft / spectrogram / welch_psd / periodogram (engine/numerics/hesap-dsp/include/crd/
 // hesap/dsp/spectral.hpp:43-222, :334-386) over its RealFftPlan, each one fused launch (spectral_kernels.hip).
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -16,9 +17,10 @@
 namespace
 {
 
-// Welch's deterministic reduction: at most this many workgroups of partial rows (fixed for a given frame count, so
-// the PSD is bit-identical run to run and device to device).
-constexpr uint64_t kWelchWorkgroups = 1024;
+// Welch: the lane-group kernel runs one resident round (CUs x workgroups per CU, at most this many workgroups); the
+// grid is fixed per device and frame count, so the PSD is bit-identical run to run. The chunked path for the
+// largest frames writes this many rows per chunk.
+constexpr uint64_t kWelchWorkgroups = 2048;
 
 crdfft_status st_of(hipError_t e)
 {
@@ -42,6 +44,7 @@ crdfft_status st_of(hipError_t e)
 struct crdfft_spectral_plan_s
 {
     uint64_t nperseg = 0, noverlap = 0, hop = 0, h = 0;
+    uint64_t welch_wgs = 0; // the resident round of the Welch kernel
     int32_t window = CRDFFT_WINDOW_HANN;
     uint32_t flags = 0;
     int e = 0;
@@ -100,7 +103,7 @@ crdfft_status run(crdfft_spectral_plan p, const float* x, uint64_t len, int mode
     const float inv = static_cast<float>(1.0 / static_cast<double>(frames));
     if (p->h <= static_cast<uint64_t>(crdfft::kSpecWelchMaxH))
     {
-        const uint64_t wgs = std::min<uint64_t>((frames + p->fpg - 1) / p->fpg, kWelchWorkgroups);
+        const uint64_t wgs = std::min<uint64_t>((frames + p->fpg - 1) / p->fpg, p->welch_wgs);
         a.out = partial;
         SPEC_TRY(crdfft::launch_spec(p->h, 2, a, wgs, s));
         SPEC_TRY(crdfft::launch_spec_sum(partial, psd, nfreq, wgs * p->fpg, true, inv, s));
@@ -175,6 +178,22 @@ crdfft_status crdfft_spectral_plan_create(crdfft_spectral_plan* plan, uint64_t n
     for (float v : w)
     {
         p->winsum2 += static_cast<double>(v) * static_cast<double>(v);
+    }
+    if (p->h <= static_cast<uint64_t>(crdfft::kSpecWelchMaxH))
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t re = hipGetDevice(&dev);
+        if (re == hipSuccess)
+            re = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (re == hipSuccess)
+            re = crdfft::spec_welch_residency(p->h, &per_cu);
+        if (re != hipSuccess)
+        {
+            delete p;
+            return st_of(re);
+        }
+        p->welch_wgs = std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(cus) * std::max(per_cu, 1),
+                                                                kWelchWorkgroups));
     }
     const uint64_t entries = crdfft::stage_twiddle_entries(p->h, e);
     hipError_t err = p->win.alloc(nperseg * sizeof(float));

@@ -29,7 +29,9 @@ constexpr int kSpecWelchMaxH = 8192;
 bool spec_shape(uint64_t h, int* e, unsigned* frames_per_group);
 // mode 0: complex rows (STFT); 1: density rows; 2: Welch partial rows, one per lane group. groups = workgroups.
 hipError_t launch_spec(uint64_t h, int mode, const SpecArgs& a, uint64_t groups, hipStream_t s);
-// acc[k] = ((first ? 0 : acc[k]) + rows[0][k] + ... + rows[nrows-1][k]) * scale, the rows in order.
+// Resident workgroups per CU of the Welch lane-group kernel (the persistent grid is this times the CU count).
+hipError_t spec_welch_residency(uint64_t h, int* per_cu);
+// acc[k] = ((first ? 0 : acc[k]) + rows[0][k] + ... + rows[nrows-1][k]) * scale, summed in a fixed order.
 hipError_t launch_spec_sum(const float* rows, float* acc, uint64_t nfreq, uint64_t nrows, bool first, float scale,
                            hipStream_t s);
 

@@ -201,20 +201,46 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
     }
 }
 
-// acc[k] = ((first ? 0 : acc[k]) + sum over r < rows of rows[r][k], in order) * scale.
-__global__ void k_spec_sum(const float* rows, float* acc, uint64_t nfreq, uint64_t nrows, int first, float scale)
+// acc[k] = ((first ? 0 : acc[k]) + sum over the rows of rows[r][k]) * scale, in a fixed order: a workgroup owns 64
+// bins, its 16 row groups sum rows g, g + 16, ... (four loads in flight per step), and the 16 partial sums are added
+// in group order, so the result does not depend on timing.
+constexpr int kSumBins = 64;
+constexpr int kSumGroups = 16;
+
+__global__ __launch_bounds__(kSumBins * kSumGroups) void k_spec_sum(const float* rows, float* acc, uint64_t nfreq,
+                                                                    uint64_t nrows, int first, float scale)
 {
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= nfreq)
+    __shared__ float part[kSumGroups][kSumBins];
+    const int kk = threadIdx.x % kSumBins;
+    const int g = threadIdx.x / kSumBins;
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kSumBins + kk;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    if (k < nfreq)
     {
-        return;
+        uint64_t r = g;
+        for (; r + 3 * kSumGroups < nrows; r += 4 * kSumGroups)
+        {
+            s0 += rows[r * nfreq + k];
+            s1 += rows[(r + kSumGroups) * nfreq + k];
+            s2 += rows[(r + 2 * kSumGroups) * nfreq + k];
+            s3 += rows[(r + 3 * kSumGroups) * nfreq + k];
+        }
+        for (; r < nrows; r += kSumGroups)
+        {
+            s0 += rows[r * nfreq + k];
+        }
     }
-    float s = first != 0 ? 0.0f : acc[k];
-    for (uint64_t r = 0; r < nrows; ++r)
+    part[g][kk] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (g == 0 && k < nfreq)
     {
-        s += rows[r * nfreq + k];
+        float s = first != 0 ? 0.0f : acc[k];
+        for (int q = 0; q < kSumGroups; ++q)
+        {
+            s += part[q][kk];
+        }
+        acc[k] = s * scale;
     }
-    acc[k] = s * scale;
 }
 
 #define CRDFFT_SPEC_LIST(X)                                                                                            \
@@ -270,11 +296,30 @@ hipError_t launch_spec(uint64_t h, int mode, const SpecArgs& a, uint64_t groups,
     return hipErrorInvalidValue;
 }
 
+hipError_t spec_welch_residency(uint64_t h, int* per_cu)
+{
+#define X(H, E)                                                                                                        \
+    if (h == H)                                                                                                        \
+    {                                                                                                                  \
+        using C = SpecCfg<H, E>;                                                                                       \
+        if (H > kSpecWelchMaxH)                                                                                        \
+            return hipErrorInvalidValue;                                                                               \
+        auto kern = k_spec<H, E, (H <= kSpecWelchMaxH ? 2 : 1)>;                                                       \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, kern, C::threads, C::lds_bytes);                   \
+    }
+    CRDFFT_SPEC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_spec_sum(const float* rows, float* acc, uint64_t nfreq, uint64_t nrows, bool first, float scale,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spec_sum, dim3(static_cast<unsigned>((nfreq + 255) / 256)), dim3(256), 0, s, rows, acc, nfreq,
-                       nrows, first ? 1 : 0, scale);
+    hipLaunchKernelGGL(k_spec_sum, dim3(static_cast<unsigned>((nfreq + kSumBins - 1) / kSumBins)),
+                       dim3(kSumBins * kSumGroups), 0, s, rows, acc, nfreq, nrows, first ? 1 : 0, scale);
     return hipGetLastError();
 }
 

@@ -331,7 +331,7 @@ CRDFFT_API crdfft_status crdfft_stft(crdfft_spectral_plan plan, const float* x, 
 CRDFFT_API crdfft_status crdfft_spectrogram(crdfft_spectral_plan plan, const float* x, uint64_t len, double fs,
                                             float* out, void* stream);
 /* welch_psd (spectral.hpp:140-172): psd = nperseg/2+1 floats, the frames' density periodograms averaged. The sum runs
- * in a fixed order (lane-group partial rows, then rows in order), so the result is bit-identical run to run. Uses
+ * in a fixed order (lane-group partial rows, then strided row groups added in group order), so the result is bit-identical run to run. Uses
  * the plan's scratch for the partial rows (per stream). All zeros when there is no frame. */
 CRDFFT_API crdfft_status crdfft_welch(crdfft_spectral_plan plan, const float* x, uint64_t len, double fs, float* psd,
                                       void* stream);
