@@ -18,6 +18,7 @@ C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
 ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED = 0, 1, 2, 3, 4, 5
 FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
+FLAG_BLUESTEIN_PASSES = 4
 ND_TRANSPOSE_ALL = 1
 WINDOW_BOXCAR, WINDOW_HANN = 0, 1
 SPECTRAL_DETREND = 1

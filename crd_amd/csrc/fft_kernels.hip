@@ -10,6 +10,8 @@
 //              and the element-major batch (fft.hpp:462-470, fft_kernels.cpp:593-784).
 //   k_rowtr    row transforms with the four-step's transposed store out[k1 + n1 k2], R rows per workgroup so every
 //              store is a run of R*8 bytes (the reference's phase 2 + scatter).
+//   k_blue     the whole Bluestein transform (chirp, FFT_m, product with FFT_m(b), IFFT_m, chirp) for m <= 16384 in
+//              the k_rows layout: one HBM read and one write per point of x (bluestein.hpp:82-135).
 //   k_roots*   FP64 table builders (the reference builds W_n in f64 and narrows, fft.hpp:251-260).
 //   k_ref_*    the radix-2 reference transform (fft.hpp:417-460), a differential check only.
 //
@@ -618,6 +620,77 @@ __global__ __launch_bounds__(TH) void k_fused(FusedArgs a)
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// k_blue: the Bluestein transform of one sequence per workgroup slot, m = N <= 16384 (the k_rows layout)
+// ---------------------------------------------------------------------------------------------------------------
+// The five passes of the chirp-z route (bluestein.hpp:82-135) with the m-point intermediate kept in registers and
+// LDS: the only HBM traffic is x in and X out (n points each) plus the L2-resident c and FFT_m(b) tables. The
+// zero padding and the k < n store guard are the buffer resources' range checks: reads at or past n return 0 and
+// writes there are dropped, as are all accesses of an inactive slot (num_records 0).
+template <int N, int E, bool INV>
+__global__ __launch_bounds__((RowCfg<N, E>::threads)) void k_blue(BlueArgs a)
+{
+    using C = RowCfg<N, E>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int local = tid / C::T;
+    const int i = tid - local * C::T;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * C::M + local;
+    const bool active = t < a.count;
+    const uint32_t bytes = static_cast<uint32_t>(a.n * sizeof(cf));
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(a.in + (active ? t * a.idist : 0), active ? bytes : 0u);
+    const __amdgpu_buffer_rsrc_t rc = make_rsrc(a.chirp, bytes);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(a.bhat, static_cast<uint32_t>(N * sizeof(cf)));
+    const uint32_t o0 = static_cast<uint32_t>(i) * sizeof(cf);
+    constexpr uint32_t kStep = C::T * sizeof(cf);
+    cf v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        cf x = bld<0>(rx, o0, m * kStep);
+        if constexpr (INV)
+        {
+            x.y = -x.y; // IDFT(x) = conj(DFT(conj x)) / n
+        }
+        v[m] = cmulc(x, bld<0>(rc, o0, m * kStep));
+    }
+    const LdsView<1> view{lds + local * C::RS, C::LF};
+    const cf* tw = reinterpret_cast<const cf*>(a.tw);
+    fft_core<N, E, false, 1>(v, i, view, tw);
+    // the table offsets pass through an empty asm after each transform, so the compiler cannot issue the table
+    // loads early and hold them in registers across the transform (at m = 16384 that spills)
+    uint32_t o1 = o0;
+    asm volatile("" : "+v"(o1));
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        v[m] = cmul(v[m], bld<0>(rb, o1, m * kStep));
+    }
+    // fft_core ends every LDS use behind a barrier, so the second transform may reuse the region at once; its lane
+    // index is laundered too, so that it recomputes its LDS addresses instead of keeping the first one's live
+    int i2 = i;
+    asm volatile("" : "+v"(i2));
+    fft_core<N, E, true, 1>(v, i2, view, tw);
+    uint32_t o2 = o0;
+    asm volatile("" : "+v"(o2));
+    const __amdgpu_buffer_rsrc_t ry = make_rsrc(a.out + (active ? t * a.odist : 0), active ? bytes : 0u);
+    const float cm = 1.0f / static_cast<float>(N);
+    const float cn = 1.0f / static_cast<float>(a.n);
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        cf y = cmulc(v[m], bld<0>(rc, o2, m * kStep));
+        y.x *= cm;
+        y.y *= cm;
+        if constexpr (INV)
+        {
+            y.x *= cn;
+            y.y = -y.y * cn;
+        }
+        bst<0>(ry, o2, m * kStep, y);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // FP64 table builders
 // ---------------------------------------------------------------------------------------------------------------
 // W_d^e as floats from FP64: the angle is reduced to (-d/2, d/2] before sincospi, so it is exact in units of pi.
@@ -768,6 +841,11 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 #define CRDFFT_FUSED_LIST(X)                                                                                           \
     X(128, 16, 256, 16, 1024) X(128, 16, 256, 16, 512) X(256, 16, 256, 16, 1024) X(256, 16, 256, 16, 512)            \
         X(256, 16, 512, 16, 1024)
+
+// (m, E) of the one-kernel Bluestein transform: the default values per thread of each convolution length
+#define CRDFFT_BLUE_LIST(X)                                                                                            \
+    X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(512, 16) X(1024, 16) X(2048, 16)           \
+        X(4096, 16) X(8192, 16) X(16384, 16)
 
 constexpr uint64_t kMaxItems = 0xFFFFFFFFull; // work-items per launch
 
@@ -1019,6 +1097,49 @@ hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, b
         return hipGetLastError();                                                                                      \
     }
     CRDFFT_FUSED_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+int blue_fused_e(uint64_t m)
+{
+#define X(N, E)                                                                                                        \
+    if (m == N)                                                                                                        \
+        return E;
+    CRDFFT_BLUE_LIST(X)
+#undef X
+    return 0;
+}
+
+hipError_t launch_blue_fused(uint64_t m, int e, bool inverse, const BlueArgs& a, hipStream_t s)
+{
+    if (a.n == 0 || 2 * a.n - 1 > m)
+    {
+        return hipErrorInvalidValue;
+    }
+#define X(N, E)                                                                                                        \
+    if (m == N && e == E)                                                                                              \
+    {                                                                                                                  \
+        using C = RowCfg<N, E>;                                                                                        \
+        auto kern = inverse ? k_blue<N, E, true> : k_blue<N, E, false>;                                                \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t max_groups = kMaxItems / C::threads;                                                            \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            BlueArgs part = a;                                                                                         \
+            const uint64_t left = a.count - done;                                                                      \
+            const uint64_t groups = std::min<uint64_t>((left + C::M - 1) / C::M, max_groups);                         \
+            part.count = std::min<uint64_t>(left, groups * C::M);                                                      \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, part);   \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_BLUE_LIST(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -134,6 +134,25 @@ hipError_t launch_blue_post(const cfloat* w, cfloat* y, const cfloat* chirp, uin
                             uint64_t odist, bool inverse, hipStream_t s);
 hipError_t build_blue_tables(cfloat* chirp, cfloat* b, uint64_t n, uint64_t m, hipStream_t s);
 
+// The whole Bluestein transform in one kernel for m <= 16384 (fft_kernels.hip): one workgroup slot per transform
+// loads x * conj(c) (zero past n), runs FFT_m, multiplies by FFT_m(b), runs the unnormalized IFFT_m and stores
+// conj(c) * w * post_scale (the inverse conjugated) for k < n. tw is the stage table of (m, e).
+struct BlueArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    const cfloat* chirp; // n entries
+    const cfloat* bhat;  // m entries
+    const cfloat* tw;
+    uint64_t n;
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+};
+// The values per thread of the one-kernel form for convolution length m, or 0 where it has no instantiation.
+int blue_fused_e(uint64_t m);
+hipError_t launch_blue_fused(uint64_t m, int e, bool inverse, const BlueArgs& a, hipStream_t s);
+
 // out[t][c][r] = in[t][r][c] (nd_kernels.hip), and data *= scale.
 hipError_t launch_transpose(const cfloat* in, cfloat* out, uint64_t rows, uint64_t cols, uint64_t count,
                             uint64_t idist, uint64_t odist, hipStream_t s);

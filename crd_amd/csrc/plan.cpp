@@ -85,6 +85,7 @@ struct crdfft_plan_s
     crdfft_plan inner = nullptr;
     DeviceBuffer chirp, bhat, bwork;
     uint64_t blue_chunk = 0;
+    int blue_e = 0; // > 0: the one-kernel form at this many values per thread (its stage table in tw_single)
     PerStream workspace_of, bwork_of; // per-stream copies of workspace / bwork
     std::mutex ref_mutex;
     ~crdfft_plan_s()
@@ -399,6 +400,15 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         const uint64_t n = sh.n, m = sh.n1;
         const cfloat* x = static_cast<const cfloat*>(in);
         cfloat* y = static_cast<cfloat*>(out);
+        if (p->blue_e > 0)
+        {
+            const crdfft::BlueArgs a{x,      y,        p->chirp.as<cfloat>(), p->bhat.as<cfloat>(), p->tw_single.as<cfloat>(),
+                                     n,      sh.batch, p->idist,              p->odist};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_blue_fused(m, p->blue_e, inverse, a, s));
+            CRDFFT_TRY(prof.after(0, n * sh.batch));
+            return CRDFFT_OK;
+        }
         cfloat* w = static_cast<cfloat*>(p->bwork_of.get(p->bwork, s));
         if (w == nullptr)
         {
@@ -594,8 +604,9 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint64_t>(batch, 1),
                                                                     crdfft::kChunkBytes / (m * 8)));
     crdfft_plan inner = nullptr;
-    crdfft_status st =
-        crdfft_plan_create(&inner, m, chunk, m, m, CRDFFT_C2C, CRDFFT_FP32, flags | CRDFFT_FLAG_NO_SELF_CHECK);
+    const int fused_e = (flags & CRDFFT_FLAG_BLUESTEIN_PASSES) != 0 ? 0 : crdfft::blue_fused_e(m);
+    crdfft_status st = crdfft_plan_create(&inner, m, fused_e > 0 ? 1 : chunk, m, m, CRDFFT_C2C, CRDFFT_FP32,
+                                          (flags & ~CRDFFT_FLAG_BLUESTEIN_PASSES) | CRDFFT_FLAG_NO_SELF_CHECK);
     if (st != CRDFFT_OK)
     {
         return st;
@@ -614,12 +625,14 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
     p->shape.algorithm = CRDFFT_ALGO_BLUESTEIN;
     p->shape.complex_n = n;
     p->shape.n1 = m;
-    p->shape.chunk = chunk;
-    p->shape.workspace_bytes = chunk * m * 8;
+    p->shape.chunk = fused_e > 0 ? std::max<uint64_t>(batch, 1) : chunk;
+    p->shape.workspace_bytes = fused_e > 0 ? 0 : chunk * m * 8;
+    p->shape.e = fused_e;
     p->idist = idist;
     p->odist = odist;
     p->flags = flags;
     p->blue_chunk = chunk;
+    p->blue_e = fused_e;
     (void)hipGetDevice(&p->device);
     auto fail = [&](crdfft_status e) {
         delete p;
@@ -629,7 +642,19 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
         return fail(hip_status(e));
     if (hipError_t e = p->bhat.alloc(m * 8); e != hipSuccess)
         return fail(hip_status(e));
-    if (hipError_t e = p->bwork.alloc(chunk * m * 8); e != hipSuccess)
+    if (fused_e > 0)
+    {
+        const uint64_t entries = crdfft::stage_twiddle_entries(m, fused_e);
+        if (entries > 0)
+        {
+            if (hipError_t e = p->tw_single.alloc(entries * sizeof(cfloat)); e != hipSuccess)
+                return fail(hip_status(e));
+            if (hipError_t e = crdfft::build_stage_twiddles(p->tw_single.as<cfloat>(), m, fused_e, nullptr);
+                e != hipSuccess)
+                return fail(hip_status(e));
+        }
+    }
+    else if (hipError_t e = p->bwork.alloc(chunk * m * 8); e != hipSuccess)
         return fail(hip_status(e));
     if (hipError_t e = crdfft::build_blue_tables(p->chirp.as<cfloat>(), p->bhat.as<cfloat>(), n, m, nullptr);
         e != hipSuccess)

@@ -228,12 +228,14 @@ class FftPlan:
 class BluesteinPlan(FftPlan):
     """Any-length C2C (bluestein.hpp:29-136): chirp-z over a power-of-two plan of m >= 2n-1; inverse scaled 1/n."""
 
-    def __init__(self, n, batch=1, idist=0, odist=0):
+    def __init__(self, n, batch=1, idist=0, odist=0, passes=False):
+        """passes=True: the five separate passes even where the one-kernel form (m <= 16384) applies."""
         self.n = int(n)
         self.batch = int(batch)
         self.type = "c2c"
         self.handle = ctypes.c_void_p()
-        check(N.lib().crdfft_bluestein_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist, 0),
+        flags = N.FLAG_BLUESTEIN_PASSES if passes else 0
+        check(N.lib().crdfft_bluestein_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist, flags),
               f"crdfft_bluestein_plan_create(n={n}, batch={batch})")
 
     def conv_size(self):

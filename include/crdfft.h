@@ -92,6 +92,8 @@ typedef enum crdfft_algorithm
 #define CRDFFT_FLAG_NONE 0u
 #define CRDFFT_FLAG_NO_SELF_CHECK 1u       /* skip the known-answer check at creation */
 #define CRDFFT_FLAG_NORMALIZE_INVERSE 2u   /* C2C inverse scaled by 1/n (ifft_normalized, fft.hpp:1015-1029) */
+#define CRDFFT_FLAG_BLUESTEIN_PASSES 4u    /* Bluestein plans: the five separate passes even where m <= 16384 would run
+                                              as one kernel (a differential check of the one-kernel form) */
 
 typedef struct crdfft_plan_s* crdfft_plan;
 
@@ -137,7 +139,9 @@ CRDFFT_API crdfft_status crdfft_plan_create_tuned(crdfft_plan* plan, uint64_t n,
 CRDFFT_API crdfft_status crdfft_plan_destroy(crdfft_plan plan);
 /* BluesteinPlan<T>(IAllocator*, usize n) (bluestein.hpp:29-70): a C2C plan of any n >= 1 run as a chirp-z
  * transform over a power-of-two plan of m >= 2n-1 (crdfft_plan_info.n1 = m). Forward unnormalized; unlike FftPlan,
- * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. */
+ * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. For
+ * m <= 16384 (n <= 8192) one kernel runs the whole transform per sequence (x read once, X written once, no
+ * workspace: crdfft_plan_info.e > 0, workspace_bytes 0); longer n run the five passes over an m-point workspace. */
 CRDFFT_API crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
                                                       uint64_t odist, uint32_t flags);
 

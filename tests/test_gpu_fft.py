@@ -268,13 +268,18 @@ def test_cpp_interface_runs(gpu, tmp_path):
     assert r.returncode == 0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 10007, 7 * 11 * 13 * 17])
-def test_bluestein_any_length(gpu, po, n):
-    """BluesteinPlan (bluestein.hpp): forward unnormalized, inverse normalized, vs FP64 and the FP32 restatement."""
+@pytest.mark.parametrize("passes", [False, True])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 8191, 8192, 10007, 7 * 11 * 13 * 17])
+def test_bluestein_any_length(gpu, po, n, passes):
+    """BluesteinPlan (bluestein.hpp): forward unnormalized, inverse normalized, vs FP64 and the FP32 restatement;
+    the one-kernel form (m <= 16384) and the five separate passes."""
     batch = 3
     x = po.lcg_complex(n * batch, 7 * n + 1).reshape(batch, n)
-    plan = gpu.BluesteinPlan(n, batch)
+    plan = gpu.BluesteinPlan(n, batch, passes=passes)
     assert plan.conv_size() >= 2 * n - 1
+    info = plan.info()
+    one_kernel = not passes and 4 <= plan.conv_size() <= 16384
+    assert (info["e"] > 0) == one_kernel and (info["workspace_bytes"] == 0) == one_kernel
     src = gpu.DeviceBuffer.from_array(x)
     dst = gpu.DeviceBuffer(x.nbytes)
     for inverse in (False, True):
@@ -291,6 +296,33 @@ def test_bluestein_any_length(gpu, po, n):
     plan.execute(src, src, gpu.FORWARD)
     gpu.synchronize()
     assert po.rel_l2(src.download(x.shape, np.complex64), po.fft64_of_f32(x)) <= 2 * bound(max(n, 2))
+
+
+@pytest.mark.parametrize("n,batch,idist,odist", [(5, 1000, 0, 0), (3, 777, 4, 6), (100, 301, 128, 100),
+                                                  (1000, 65, 1000, 1024), (6000, 9, 6001, 6000)])
+def test_bluestein_one_kernel_batches(gpu, po, n, batch, idist, odist):
+    """The one-kernel form over many sequences per workgroup with a partial last workgroup, and padded layouts:
+    equal to the five-pass form within FP32 rounding, and nothing outside [0, n) of an output row is written."""
+    idist, odist = idist or n, odist or n
+    rng = np.random.default_rng(n + batch)
+    xin = (rng.standard_normal((batch, idist)) + 1j * rng.standard_normal((batch, idist))).astype(np.complex64)
+    src = gpu.DeviceBuffer.from_array(xin)
+    guard = np.full((batch, odist), 7.0 + 3.0j, dtype=np.complex64)
+    outs = []
+    for passes in (False, True):
+        plan = gpu.BluesteinPlan(n, batch, idist, odist, passes=passes)
+        dst = gpu.DeviceBuffer.from_array(guard)
+        for inverse in (False, True):
+            plan.execute(src, dst, gpu.INVERSE if inverse else gpu.FORWARD)
+            gpu.synchronize()
+            y = dst.download(guard.shape, np.complex64)
+            assert np.all(y[:, n:] == guard[:, n:])
+            ref = po.fft64_of_f32(xin[:, :n], inverse=inverse)
+            if inverse:
+                ref = ref / n
+            assert po.rel_l2(y[:, :n], ref) <= 2 * bound(n), (passes, inverse)
+            outs.append(y[:, :n])
+    assert po.rel_l2(outs[0], outs[2]) <= 2 * bound(n) and po.rel_l2(outs[1], outs[3]) <= 2 * bound(n)
 
 
 @pytest.mark.parametrize("dims", [(8, 16), (64, 64, 64), (3, 5, 7), (16, 1, 32), (5, 64), (64, 5), (2, 8192, 4),
