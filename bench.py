@@ -241,6 +241,41 @@ def bench_spectral(stream, Event):
     return out
 
 
+def bench_r2r(stream, Event):
+    """DCT-II/III and DST-II/III of 4096 rows of 4096 FP32 samples (64 MiB in, 64 MiB out): the one-kernel form and
+    the separate passes around the R2C / C2R plans; GB/s counts one read and one write of every sample. Then one
+    any-length C2C, n = 1000 (Bluestein, m = 2048) over 16777 rows: the one-kernel form against the five passes."""
+    import crd_amd
+    from crd_amd import DeviceBuffer
+    from crd_amd.dct import DctPlan
+
+    n, rows = 4096, 4096
+    x = np.random.default_rng(12).standard_normal((rows, n), dtype=np.float32)
+    src, dst = DeviceBuffer.from_array(x), DeviceBuffer(x.nbytes)
+    out = {"n": n, "rows": rows}
+    names = ("dct2", "dct3", "dst2", "dst3")
+    for passes in (False, True):
+        plan = DctPlan(n, rows, passes=passes)
+        form = {}
+        for kind in range(4):
+            ms = time_loop(lambda: plan.execute(kind, src, dst, stream), stream, 20, 3, Event)
+            form[names[kind]] = {"ms": round(ms, 4), "gbs": round(8.0 * n * rows / (ms * 1e-3) / 1e9, 1)}
+        out["separate_passes" if passes else "one_kernel"] = form
+    out["one_kernel_vs_passes_dct2"] = round(out["separate_passes"]["dct2"]["ms"] / out["one_kernel"]["dct2"]["ms"], 2)
+    bn, bb = 1000, 16777
+    xb = np.random.default_rng(13).standard_normal((bb, 2 * bn), dtype=np.float32).view(np.complex64)
+    bsrc, bdst = DeviceBuffer.from_array(xb), DeviceBuffer(xb.nbytes)
+    blue = {"n": bn, "batch": bb}
+    for passes in (False, True):
+        plan = crd_amd.BluesteinPlan(bn, bb, passes=passes)
+        ms = time_loop(lambda: plan.execute(bsrc, bdst, crd_amd.FORWARD, stream), stream, 20, 3, Event)
+        blue["separate_passes" if passes else "one_kernel"] = {"ms": round(ms, 4),
+                                                               "gbs": round(16.0 * bn * bb / (ms * 1e-3) / 1e9, 1)}
+    blue["one_kernel_vs_passes"] = round(blue["separate_passes"]["ms"] / blue["one_kernel"]["ms"], 2)
+    out["bluestein"] = blue
+    return out
+
+
 def bench_split(stream, Event):
     """n = 2^16, batch 2048, |x| log-uniform in [1e-4, 1e4] with uniform phase: plain FP32 against the split-BF16
     and split-FP16 MFMA forms, time and rel-L2 against FP64 (numpy, float64) on a sample."""
@@ -480,6 +515,7 @@ def main():
         extras["mixed_size"] = bench_mixed(stream, Event)
         extras["split_precision"] = bench_split(stream, Event)
         extras["spectral"] = bench_spectral(stream, Event)
+        extras["r2r_and_bluestein"] = bench_r2r(stream, Event)
         result["extras"] = extras
 
     if not args.no_cpu:

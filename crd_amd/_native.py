@@ -18,7 +18,8 @@ C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
 ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED = 0, 1, 2, 3, 4, 5
 FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
-FLAG_BLUESTEIN_PASSES = 4
+FLAG_SEPARATE_PASSES = 4
+DCT2, DCT3, DST2, DST3 = 0, 1, 2, 3
 ND_TRANSPOSE_ALL = 1
 WINDOW_BOXCAR, WINDOW_HANN = 0, 1
 SPECTRAL_DETREND = 1
@@ -178,6 +179,10 @@ _SIGS = {
     "crdfft_many_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_many_graph_capture": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_many_graph_launch": (_i32, [_vp, _vp]),
+    "crdfft_dct_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _u32]),
+    "crdfft_dct_plan_destroy": (_i32, [_vp]),
+    "crdfft_dct_plan_get_info": (_i32, [_vp, ctypes.POINTER(PlanInfo)]),
+    "crdfft_dct_execute": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "crdfft_spectral_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _i32, _u32]),
     "crdfft_spectral_plan_destroy": (_i32, [_vp]),
     "crdfft_spectral_frames": (_u64, [_vp, _u64]),

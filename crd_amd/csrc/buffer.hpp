@@ -35,6 +35,28 @@ template <int AUX> __device__ __forceinline__ float bld32(__amdgpu_buffer_rsrc_t
     return __builtin_bit_cast(float, u);
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int AUX> __device__ __forceinline__ f4v bld128(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    const auto u = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(voff), static_cast<int>(soff), AUX);
+    static_assert(sizeof(u) == 16, "128-bit buffer access");
+    return __builtin_bit_cast(f4v, u);
+}
+
+template <int AUX> __device__ __forceinline__ void bst32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, static_cast<int>(voff),
+                                          static_cast<int>(soff), AUX);
+}
+
+template <int AUX> __device__ __forceinline__ void bst128(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, f4v v)
+{
+    using u4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, static_cast<int>(voff), static_cast<int>(soff),
+                                           AUX);
+}
+
 template <int AUX> __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, cf v)
 {
     f2v f;

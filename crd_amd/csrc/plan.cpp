@@ -604,9 +604,9 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint64_t>(batch, 1),
                                                                     crdfft::kChunkBytes / (m * 8)));
     crdfft_plan inner = nullptr;
-    const int fused_e = (flags & CRDFFT_FLAG_BLUESTEIN_PASSES) != 0 ? 0 : crdfft::blue_fused_e(m);
+    const int fused_e = (flags & CRDFFT_FLAG_SEPARATE_PASSES) != 0 ? 0 : crdfft::blue_fused_e(m);
     crdfft_status st = crdfft_plan_create(&inner, m, fused_e > 0 ? 1 : chunk, m, m, CRDFFT_C2C, CRDFFT_FP32,
-                                          (flags & ~CRDFFT_FLAG_BLUESTEIN_PASSES) | CRDFFT_FLAG_NO_SELF_CHECK);
+                                          (flags & ~CRDFFT_FLAG_SEPARATE_PASSES) | CRDFFT_FLAG_NO_SELF_CHECK);
     if (st != CRDFFT_OK)
     {
         return st;

@@ -234,7 +234,7 @@ class BluesteinPlan(FftPlan):
         self.batch = int(batch)
         self.type = "c2c"
         self.handle = ctypes.c_void_p()
-        flags = N.FLAG_BLUESTEIN_PASSES if passes else 0
+        flags = N.FLAG_SEPARATE_PASSES if passes else 0
         check(N.lib().crdfft_bluestein_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist, flags),
               f"crdfft_bluestein_plan_create(n={n}, batch={batch})")
 

@@ -92,8 +92,8 @@ typedef enum crdfft_algorithm
 #define CRDFFT_FLAG_NONE 0u
 #define CRDFFT_FLAG_NO_SELF_CHECK 1u       /* skip the known-answer check at creation */
 #define CRDFFT_FLAG_NORMALIZE_INVERSE 2u   /* C2C inverse scaled by 1/n (ifft_normalized, fft.hpp:1015-1029) */
-#define CRDFFT_FLAG_BLUESTEIN_PASSES 4u    /* Bluestein plans: the five separate passes even where m <= 16384 would run
-                                              as one kernel (a differential check of the one-kernel form) */
+#define CRDFFT_FLAG_SEPARATE_PASSES 4u     /* Bluestein and DCT plans: the separate passes even where one kernel would
+                                              do (a differential check of the one-kernel forms) */
 
 typedef struct crdfft_plan_s* crdfft_plan;
 
@@ -339,6 +339,36 @@ CRDFFT_API crdfft_status crdfft_spectrogram(crdfft_spectral_plan plan, const flo
  * the plan's scratch for the partial rows (per stream). All zeros when there is no frame. */
 CRDFFT_API crdfft_status crdfft_welch(crdfft_spectral_plan plan, const float* x, uint64_t len, double fs, float* psd,
                                       void* stream);
+
+/* ---- real-to-real transforms (DctPlan<f32>, hesap-fft dct.hpp:34-170) ----
+ * A plan of `batch` rows of n reals (n a power of two, 2 <= n <= 2^25), row t at in + t*idist and out + t*odist
+ * (0 = packed, n). Conventions of scipy norm=None (dct.hpp:10-17):
+ *   DCT-II : y[k] = 2 sum_n x[n] cos(pi (2n+1) k / 2N)
+ *   DCT-III: y[k] = x[0] + 2 sum_{n>=1} x[n] cos(pi n (2k+1) / 2N)            (dct3(dct2(x)) = 2N x)
+ *   DST-II : y[k] = 2 sum_n x[n] sin(pi (2n+1) (k+1) / 2N)
+ *   DST-III: y[k] = (-1)^k x[N-1] + 2 sum_{n<=N-2} x[n] sin(pi (n+1) (2k+1) / 2N)  (dst3(dst2(x)) = 2N x)
+ * computed by Makhoul's reduction to one complex FFT of n/2 points. n <= 32768 runs one kernel per call (each row
+ * read once and written once, no workspace); longer n, or CRDFFT_FLAG_SEPARATE_PASSES, run a shuffle / rotation
+ * pass, the library's R2C or C2R plan and an output pass over a chunked workspace (per stream). in == out is
+ * allowed when idist == odist. */
+typedef struct crdfft_dct_plan_s* crdfft_dct_plan;
+typedef enum crdfft_r2r_kind
+{
+    CRDFFT_DCT2 = 0, /* DctPlan::dct2 */
+    CRDFFT_DCT3 = 1, /* DctPlan::dct3 */
+    CRDFFT_DST2 = 2, /* DctPlan::dst2 */
+    CRDFFT_DST3 = 3  /* DctPlan::dst3 */
+} crdfft_r2r_kind;
+/* DctPlan<T>(IAllocator*, usize n) (dct.hpp:38-57), for a batch of rows. flags: CRDFFT_FLAG_NO_SELF_CHECK,
+ * CRDFFT_FLAG_SEPARATE_PASSES. */
+CRDFFT_API crdfft_status crdfft_dct_plan_create(crdfft_dct_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
+                                                uint64_t odist, uint32_t flags);
+CRDFFT_API crdfft_status crdfft_dct_plan_destroy(crdfft_dct_plan plan);
+/* n, batch, complex_n = n/2, e (> 0: the one-kernel form), workspace and table bytes. */
+CRDFFT_API crdfft_status crdfft_dct_plan_get_info(crdfft_dct_plan plan, crdfft_plan_info* info);
+/* DctPlan::dct2 / dct3 / dst2 / dst3 (dct.hpp:62-170) of every row, asynchronously on `stream`. */
+CRDFFT_API crdfft_status crdfft_dct_execute(crdfft_dct_plan plan, int32_t kind, const float* in, float* out,
+                                            void* stream);
 
 CRDFFT_API const char* crdfft_status_string(crdfft_status status);
 CRDFFT_API int32_t crdfft_last_hip_error(void);

@@ -23,6 +23,9 @@
  *                    and a planted wrong twiddle sign that the tests must catch.
  *   oracle_*32_batch bit-identity across thread counts (batch_arms.hpp:11-14) and FP64 truth.
  *   oracle_bluestein32   FP64 truth for primes and 7-smooth n.
+ *   oracle_dct_direct64 / oracle_dct64   scipy.fft.dct/dst types 2 and 3 (norm=None, the reference's stated
+ *                    convention), each other, closed forms (constant input, a single cosine), and the inverse pairs
+ *                    dct3(dct2(x)) = dst3(dst2(x)) = 2N x.
  *   oracle_welch64 / oracle_periodograms64   the reference's scipy vectors (spectral_refs.inc): Welch, spectrogram,
  *                    periodogram; oracle_stft64 numpy rfft of explicitly windowed frames; oracle_hann64 the closed form
  *                    and scipy's hann.
@@ -91,6 +94,13 @@ size_t oracle_periodograms64(const double* x, size_t len, double fs, size_t nper
 size_t oracle_welch64(const double* x, size_t len, double fs, size_t nperseg, size_t noverlap, double* psd);
 /* STFT: complex rfft of every hann-windowed frame, no detrend, no scaling (spectral.hpp:334-386). */
 size_t oracle_stft64(const double* x, size_t len, size_t nperseg, size_t noverlap, double* out);
+
+/* ---- FP64 real-to-real transforms (dct.hpp definitions, scipy norm=None; dct.cpp). kind: 0 DCT-II, 1 DCT-III,
+ * 2 DST-II, 3 DST-III. ---- */
+/* The direct O(n^2) sums (dct.hpp:171-239), any n >= 1. */
+void oracle_dct_direct64(const double* x, double* y, size_t n, int kind);
+/* The same transforms through a 2n-point FP64 FFT of the padded, phase-rotated input; n a power of two; rows of n. */
+void oracle_dct64(const double* x, double* y, size_t n, size_t batch, int kind);
 
 /* ---- FP64 restatement of the plan cost model (mlp.cpp): 3-layer ReLU perceptron, flat parameters
  * W1[h][in], b1[h], W2[h][h], b2[h], w3[h], b3; pinned by central finite differences. ---- */

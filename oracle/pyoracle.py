@@ -52,6 +52,8 @@ def lib():
             "oracle_periodograms64": (sz, [vp, sz, ctypes.c_double, sz, sz, i, i, vp]),
             "oracle_welch64": (sz, [vp, sz, ctypes.c_double, sz, sz, vp]),
             "oracle_stft64": (sz, [vp, sz, sz, sz, vp]),
+            "oracle_dct_direct64": (None, [vp, vp, sz, i]),
+            "oracle_dct64": (None, [vp, vp, sz, sz, i]),
             "oracle_mlp_forward": (None, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp]),
             "oracle_mlp_loss_grad": (ctypes.c_double, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp]),
         }
@@ -73,6 +75,13 @@ def lcg_complex(count, seed, dtype=np.complex64):
     lib().oracle_lcg_fill64(_p(out), out.size, seed & 0xFFFFFFFFFFFFFFFF)
     c = out.view(np.complex128)
     return c.astype(dtype) if dtype != np.complex128 else c
+
+
+def lcg_real(count, seed, dtype=np.float64):
+    """`count` reals from the same LCG stream (the reference's DCT tests fill rows this way, test_dct.cpp:23-31)."""
+    out = np.empty(count, dtype=np.float64)
+    lib().oracle_lcg_fill64(_p(out), out.size, seed & 0xFFFFFFFFFFFFFFFF)
+    return out.astype(dtype) if dtype != np.float64 else out
 
 
 def normal_real(count, seed):
@@ -154,6 +163,26 @@ def stft64(x, nperseg, noverlap):
     out = np.empty((max(nf, 1), nperseg // 2 + 1), dtype=np.complex128)
     got = lib().oracle_stft64(_p(x), len(x), nperseg, noverlap, _p(out))
     return out[:got]
+
+
+DCT2, DCT3, DST2, DST3 = 0, 1, 2, 3
+
+
+def dct_direct64(x, kind):
+    """The direct O(n^2) sum of one real sequence (FP64)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().oracle_dct_direct64(_p(x), _p(y), len(x), int(kind))
+    return y
+
+
+def dct64(x, kind):
+    """Rows of a pow2 length through the 2n-point FP64 FFT route."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    y = np.empty_like(x)
+    lib().oracle_dct64(_p(x), _p(y), n, x.size // n, int(kind))
+    return y
 
 
 def irfft64(X, n):

@@ -237,3 +237,57 @@ def test_hann_and_stft_oracle(po):
     # a signal shorter than one frame has no frames, and Welch of it is all zeros
     assert po.stft64(x[:63], 64, 32).shape[0] == 0
     assert np.all(po.welch64(x[:63], 1.0, 64) == 0)
+
+
+KINDS = ("dct2", "dct3", "dst2", "dst3")
+
+
+def scipy_r2r(x, kind):
+    import scipy.fft as sf
+
+    return {0: lambda v: sf.dct(v, 2), 1: lambda v: sf.dct(v, 3), 2: lambda v: sf.dst(v, 2),
+            3: lambda v: sf.dst(v, 3)}[kind](x)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 8, 16, 64, 100, 256, 1024])
+def test_dct_oracle_matches_scipy(po, n):
+    """dct.hpp states its conventions as scipy norm=None (dct.hpp:5-17, verified there against scipy): the direct sums
+    and the 2N-point FFT route both equal scipy.fft.dct/dst types 2 and 3 at the reference's f64 gate (1e-12 of
+    1 + max|y|, test_dct.cpp:83-94), on the reference tests' LCG rows."""
+    for kind in range(4):
+        x = po.lcg_real(n, 0xC7C7 ^ (n * 7 + kind))
+        ref = scipy_r2r(x, kind)
+        d = po.dct_direct64(x, kind)
+        assert np.max(np.abs(d - ref)) / (1 + np.max(np.abs(ref))) < 1e-12, (KINDS[kind], n)
+        if n >= 2 and (n & (n - 1)) == 0:
+            f = po.dct64(x[None, :], kind)[0]
+            assert np.max(np.abs(f - ref)) / (1 + np.max(np.abs(ref))) < 1e-12, (KINDS[kind], n)
+
+
+def test_dct_oracle_closed_forms_and_inverse_pairs(po):
+    """Closed forms: DCT-II of a constant row c is (2Nc, 0, ...); DCT-II of cos(pi (2n+1) k0 / 2N) is N at k0 and 0
+    elsewhere; DST-II of sin(pi (2n+1)(k0+1) / 2N) is N at k0 (2N at k0 = N-1); DCT-III of a unit impulse is all ones. Inverse pairs:
+    dct3(dct2(x)) = dst3(dst2(x)) = 2N x (test_dct.cpp:97-131). A transposed pair (dct3 where dct2 belongs) or a
+    dropped half-sample phase fails these."""
+    for n in (4, 64, 1024):
+        i = np.arange(n)
+        y = po.dct64(np.full((1, n), 0.75), 0)[0]
+        assert abs(y[0] - 2 * n * 0.75) < 1e-12 * n and np.max(np.abs(y[1:])) < 1e-12 * n
+        for k0 in (1, n // 3, n - 1):
+            y = po.dct64(np.cos(np.pi * (2 * i + 1) * k0 / (2 * n))[None, :], 0)[0]
+            want = np.zeros(n)
+            want[k0] = n
+            assert np.max(np.abs(y - want)) < 1e-10 * n
+            y = po.dct64(np.sin(np.pi * (2 * i + 1) * (k0 + 1) / (2 * n))[None, :], 2)[0]
+            want = np.zeros(n)
+            want[k0] = 2 * n if k0 == n - 1 else n  # k0 + 1 = N: the row is (-1)^n, sum of squares N, not N/2
+            assert np.max(np.abs(y - want)) < 1e-10 * n
+        imp = np.zeros((1, n))
+        imp[0, 0] = 1.0
+        assert np.allclose(po.dct64(imp, 1)[0], 1.0, atol=1e-14, rtol=0)
+        x = po.lcg_real(n, 0x1234 ^ n)[None, :]
+        for fwd, inv in ((0, 1), (2, 3)):
+            r = po.dct64(po.dct64(x, fwd), inv)
+            assert np.max(np.abs(r - 2 * n * x)) / (1 + 2 * n) < 1e-12
+        # the wrong inverse does not round-trip
+        assert np.max(np.abs(po.dct64(po.dct64(x, 0), 3) - 2 * n * x)) > 1e-3
