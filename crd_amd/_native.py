@@ -16,9 +16,11 @@ OK, INVALID_SIZE, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, SELF_CHECK_FAILED,
 FORWARD, INVERSE = 0, 1
 C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
-ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED = 0, 1, 2, 3, 4, 5
+ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUSED, ALGO_COOP = 0, 1, 2, 3, 4, 5, 6
+HINT_COOP_SOLO = 16
 FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
 FLAG_SEPARATE_PASSES = 4
+HINTS_NONE = 0x100
 DCT2, DCT3, DST2, DST3 = 0, 1, 2, 3
 ND_TRANSPOSE_ALL = 1
 WINDOW_BOXCAR, WINDOW_HANN = 0, 1
@@ -138,6 +140,7 @@ _SIGS = {
     "crdfft_plan_create_tuned": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _i32, _i32, _u32,
                                         ctypes.POINTER(Tuning)]),
     "crdfft_plan_destroy": (_i32, [_vp]),
+    "crdfft_plan_coop_status": (_i32, [_vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "crdfft_bluestein_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _u32]),
     "crdfft_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_execute_batched": (_i32, [_vp, _vp, _u64, _i32, _vp]),

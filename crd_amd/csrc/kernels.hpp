@@ -89,6 +89,32 @@ struct FusedArgs
     uint32_t hints;
 };
 
+// One square transform n = n1 * n1 split over a group of workgroups of one XCD (k_coop): each member runs one
+// column tile and one row tile of every transform its group takes, and the group's n-point slot is shared through
+// that XCD's L2. Groups form at launch from the XCC_ID each workgroup reads; a launch that does not become resident
+// as a whole (or kHintCoopSolo) runs every workgroup alone on its own slot.
+struct CoopArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    cfloat* slots;     // gridDim.x slots of n points
+    uint32_t* ctl;     // coop_ctl_words(grid) control words, cleared by launch_coop
+    const cfloat* tw;  // stage twiddles of n1
+    const cfloat* thi; // W_n^{n1 j}, j < n1
+    const cfloat* tlo; // W_n^{j}, j < n1
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+    float scale;
+    uint32_t hints;
+    uint32_t spin_limit; // polls (each followed by a short sleep) before a wait gives up and flags an error
+};
+constexpr uint32_t kHintCoopSolo = 16u;
+// control words: grid arrival, mode, error, 8 per-XCD ticket counters, then two barrier counters per unit (each on
+// its own 128-byte line)
+constexpr uint32_t kCoopCtlGrid = 0, kCoopCtlMode = 32, kCoopCtlError = 64, kCoopCtlXcd = 96, kCoopCtlUnits = 352;
+inline uint64_t coop_ctl_words(uint64_t grid) { return kCoopCtlUnits + 64 * grid; }
+
 struct LaunchShape
 {
     unsigned threads = 0;     // workgroup size
@@ -102,6 +128,12 @@ bool col_kernel_shape(uint64_t n1, int e, int c, LaunchShape* out);
 bool rowtr_kernel_shape(uint64_t n2, int e, int r, LaunchShape* out);
 
 bool fused_kernel_shape(uint64_t n1, int e1, uint64_t n2, int e2, int threads, LaunchShape* out);
+bool coop_kernel_shape(uint64_t n1, int e, int threads, LaunchShape* out);
+// Resident workgroups per CU of the cooperative kernel (sets its LDS attribute first).
+hipError_t coop_blocks_per_cu(uint64_t n1, int e, int threads, int* blocks);
+// Clears the control words and launches `grid` workgroups (grid <= the slots behind a.slots).
+hipError_t launch_coop(uint64_t n1, int e, int threads, bool inverse, const CoopArgs& a, unsigned grid,
+                       hipStream_t s);
 // Resident workgroups per CU of the fused kernel (sets its LDS attribute first).
 hipError_t fused_blocks_per_cu(uint64_t n1, int e1, uint64_t n2, int e2, int threads, int* blocks);
 

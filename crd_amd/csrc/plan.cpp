@@ -85,6 +85,7 @@ struct crdfft_plan_s
     crdfft_plan inner = nullptr;
     DeviceBuffer chirp, bhat, bwork;
     uint64_t blue_chunk = 0;
+    uint64_t coop_ctl_offset = 0; // COOP: byte offset of the control words in the workspace
     int blue_e = 0; // > 0: the one-kernel form at this many values per thread (its stage table in tw_single)
     PerStream workspace_of, bwork_of; // per-stream copies of workspace / bwork
     std::mutex ref_mutex;
@@ -226,6 +227,29 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
         p->shape.workspace_bytes = p->shape.chunk * cn * sizeof(cfloat);
         CRDFFT_TRY(p->workspace.alloc(p->shape.workspace_bytes));
     }
+    else if (sh.algorithm == CRDFFT_ALGO_COOP)
+    {
+        const uint64_t e1n = crdfft::stage_twiddle_entries(sh.n1, sh.e1);
+        CRDFFT_TRY(p->tw1.alloc(e1n * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_stage_twiddles(p->tw1.as<cfloat>(), sh.n1, sh.e1, s));
+        CRDFFT_TRY(p->thi.alloc(sh.n1 * sizeof(cfloat)));
+        CRDFFT_TRY(p->tlo.alloc(sh.n1 * sizeof(cfloat)));
+        CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), sh.n1, sh.n1, cn, s));
+        CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), sh.n1, 1, cn, s));
+        // one workgroup (and slot) per resident workgroup of this device, so the launch can be resident as a whole
+        int per_cu = 0;
+        hipDeviceProp_t d{};
+        CRDFFT_TRY(crdfft::coop_blocks_per_cu(sh.n1, sh.e1, sh.threads, &per_cu));
+        CRDFFT_TRY(hipGetDeviceProperties(&d, p->device));
+        if (per_cu <= 0)
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+        p->shape.chunk = static_cast<uint64_t>(d.multiProcessorCount) * static_cast<uint64_t>(per_cu);
+        p->coop_ctl_offset = (p->shape.chunk * cn * sizeof(cfloat) + 255) / 256 * 256;
+        p->shape.workspace_bytes = p->coop_ctl_offset + crdfft::coop_ctl_words(p->shape.chunk) * sizeof(uint32_t);
+        CRDFFT_TRY(p->workspace.alloc(p->shape.workspace_bytes));
+    }
     else if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP)
     {
         const uint64_t e1n = crdfft::stage_twiddle_entries(sh.n1, sh.e1);
@@ -323,6 +347,31 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                 CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
                 CRDFFT_TRY(prof.after(1, cn * cnt));
             }
+            return CRDFFT_OK;
+        }
+        case CRDFFT_ALGO_COOP:
+        {
+            char* ws = static_cast<char*>(p->workspace_of.get(p->workspace, s));
+            if (ws == nullptr)
+            {
+                return CRDFFT_OUT_OF_MEMORY;
+            }
+            const crdfft::CoopArgs ca{in,
+                                      out,
+                                      reinterpret_cast<cfloat*>(ws),
+                                      reinterpret_cast<uint32_t*>(ws + p->coop_ctl_offset),
+                                      p->tw1.as<cfloat>(),
+                                      p->thi.as<cfloat>(),
+                                      p->tlo.as<cfloat>(),
+                                      count,
+                                      idist,
+                                      odist,
+                                      scale,
+                                      sh.hints,
+                                      crdfft::kCoopSpinLimit};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_coop(sh.n1, sh.e1, sh.threads, inverse, ca, static_cast<unsigned>(sh.chunk), s));
+            CRDFFT_TRY(prof.after(0, cn * count));
             return CRDFFT_OK;
         }
         case CRDFFT_ALGO_FUSED:
@@ -667,6 +716,32 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
     if (hipError_t e = hipStreamSynchronize(nullptr); e != hipSuccess)
         return fail(hip_status(e));
     *plan = p;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_plan_coop_status(crdfft_plan p, void* stream, uint32_t* waits_given_up, uint32_t* mode)
+{
+    if (p == nullptr || waits_given_up == nullptr || mode == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *waits_given_up = 0;
+    *mode = 0;
+    if (p->shape.algorithm != CRDFFT_ALGO_COOP)
+    {
+        return CRDFFT_OK;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(p->workspace_of.get(p->workspace, s));
+    if (ws == nullptr)
+    {
+        return CRDFFT_OUT_OF_MEMORY;
+    }
+    CRDFFT_TRY(hipStreamSynchronize(s));
+    uint32_t words[96];
+    CRDFFT_TRY(hipMemcpy(words, ws + p->coop_ctl_offset, sizeof(words), hipMemcpyDeviceToHost));
+    *waits_given_up = words[crdfft::kCoopCtlError];
+    *mode = words[crdfft::kCoopCtlMode];
     return CRDFFT_OK;
 }
 

@@ -72,5 +72,9 @@ bool have_rowtr_kernel(uint64_t n2, int e, int r);
 bool have_cols_pf(uint64_t n1, int e, int c);
 // Fused four-step of n1 x n2 at threads per workgroup; lds_bytes receives its LDS per workgroup.
 bool have_fused_kernel(uint64_t n1, int e1, uint64_t n2, int e2, int threads, unsigned* lds_bytes = nullptr);
+// The cooperative kernel of n = n1^2 (at *e values per thread, or the first instantiated when *e is 0): its values
+// per thread and workgroup size, false where none is instantiated.
+bool coop_shape_of(uint64_t n, uint64_t* n1, int* e, int* threads, unsigned* lds_bytes);
+constexpr uint32_t kCoopSpinLimit = 1u << 18;
 
 } // namespace crdfft

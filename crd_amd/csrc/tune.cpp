@@ -35,7 +35,7 @@ crdfft_tuning tuning_of(const crdfft::PlanShape& s)
     t.e2 = s.e2;
     t.tile2 = s.tile2;
     t.chunk = s.chunk;
-    t.hints = s.hints;
+    t.hints = s.hints == 0 ? CRDFFT_HINTS_NONE : s.hints; // a tuning's 0 would mean the planner's default
     t.algorithm = s.algorithm;
     return t;
 }
@@ -71,18 +71,33 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
     {
         for (int e : {8, 16, 32, 64})
         {
-            for (uint32_t hints : {0u, 3u})
+            for (uint32_t hints : {CRDFFT_HINTS_NONE, 3u})
             {
                 crdfft_tuning t{};
                 t.e = e;
-                t.hints = hints == 0 ? 0 : hints; // 0 keeps the default (none)
+                t.hints = hints;
                 add(&t);
             }
         }
     }
-    else if (def.algorithm == CRDFFT_ALGO_FOUR_STEP || def.algorithm == CRDFFT_ALGO_FUSED)
+    else if (def.algorithm == CRDFFT_ALGO_FOUR_STEP || def.algorithm == CRDFFT_ALGO_FUSED ||
+             def.algorithm == CRDFFT_ALGO_COOP)
     {
         const uint32_t lg = crdfft::log2_exact(cn);
+        // the cooperative single-pass form where it is instantiated
+        uint64_t cn1 = 0;
+        int ce = 0, cth = 0; // the first instantiated shape
+        unsigned clds = 0;
+        if (crdfft::coop_shape_of(cn, &cn1, &ce, &cth, &clds))
+        {
+            for (uint32_t hints : {1u, 3u})
+            {
+                crdfft_tuning t{};
+                t.algorithm = CRDFFT_ALGO_COOP;
+                t.hints = hints;
+                add(&t);
+            }
+        }
         // the fused four-step at its instantiated workgroup sizes (tile1 = columns per phase-A tile)
         for (int tile1 : {0, 32, 64, 128})
         {
@@ -245,7 +260,7 @@ crdfft_status crdfft_time_plan(uint64_t n, uint64_t batch, int32_t type, const c
     record->tuning.e2 = info.e2;
     record->tuning.tile2 = info.tile2;
     record->tuning.chunk = info.chunk;
-    record->tuning.hints = info.hints;
+    record->tuning.hints = info.hints == 0 ? CRDFFT_HINTS_NONE : info.hints; // replayable: 0 means the default
     record->tuning.algorithm = info.algorithm;
     record->algorithm = info.algorithm;
     record->ms = ms[ms.size() / 2];
@@ -349,7 +364,7 @@ crdfft_status crdfft_plan_features(const crdfft_plan_record* records, uint64_t c
         f[28] = r.ndim >= 2 ? log2f_safe(static_cast<double>(r.dims[0])) / 12.0f : 0.0f;
         f[29] = r.ndim >= 3 ? log2f_safe(static_cast<double>(r.dims[1])) / 12.0f : 0.0f;
         f[30] = r.algorithm == CRDFFT_ALGO_FOUR_STEP ? 1.0f : 0.0f;
-        f[31] = r.algorithm == CRDFFT_ALGO_FUSED ? 1.0f : 0.0f;
+        f[31] = r.algorithm == CRDFFT_ALGO_FUSED ? 1.0f : (r.algorithm == CRDFFT_ALGO_COOP ? 0.5f : 0.0f);
     }
     return CRDFFT_OK;
 }

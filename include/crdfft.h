@@ -84,8 +84,11 @@ typedef enum crdfft_algorithm
     CRDFFT_ALGO_FOUR_STEP = 2,  /* n = n1*n2: column pass with twiddle, row pass with transposed store */
     CRDFFT_ALGO_REFERENCE = 3,  /* the radix-2 reference only */
     CRDFFT_ALGO_BLUESTEIN = 4,  /* chirp-z over a power-of-two plan of m >= 2n-1 (any n) */
-    CRDFFT_ALGO_FUSED = 5       /* n = n1*n2 <= 2^17: both four-step passes of a transform in one workgroup of a
+    CRDFFT_ALGO_FUSED = 5,      /* n = n1*n2 <= 2^17: both four-step passes of a transform in one workgroup of a
                                    persistent launch, the intermediate in a per-workgroup slot (chunk = slots) */
+    CRDFFT_ALGO_COOP = 6        /* n = n1*n1 (4096, 65536): both passes of a transform split over a group of
+                                   workgroups of one XCD, the intermediate shared through that XCD's L2
+                                   (chunk = workgroups of the launch) */
 } crdfft_algorithm;
 
 /* Plan flags */
@@ -117,15 +120,19 @@ typedef struct crdfft_plan_info
 } crdfft_plan_info;
 
 /* Tuning overrides for crdfft_plan_create_tuned; a zero field keeps the planner's choice. */
+#define CRDFFT_HINTS_NONE 0x100u /* crdfft_tuning.hints: no cache hints (plain loads and stores) */
 typedef struct crdfft_tuning
 {
     uint64_t n1;          /* four-step split */
     int32_t e, e1, tile1, e2, tile2;
     uint64_t chunk;       /* transforms per pass pair */
-    uint32_t hints;       /* cache hints (crdfft_plan_info.hints); 0 keeps the planner's */
-    int32_t algorithm;    /* CRDFFT_ALGO_FOUR_STEP or CRDFFT_ALGO_FUSED for n > 16384; 0 keeps the planner's.
+    uint32_t hints;       /* cache hints (crdfft_plan_info.hints); 0 keeps the planner's, CRDFFT_HINTS_NONE
+                             asks for none */
+    int32_t algorithm;    /* CRDFFT_ALGO_FOUR_STEP, CRDFFT_ALGO_FUSED or CRDFFT_ALGO_COOP for n > 16384 (COOP also
+                             at 4096); 0 keeps the planner's.
                              FUSED: tile1 = columns per phase-A tile (threads = n1/e1 * tile1), tile2 derived,
-                             chunk caps the persistent workgroups (slots) */
+                             chunk caps the persistent workgroups (slots).
+                             COOP: hints bit 16 runs every workgroup alone (a differential check) */
 } crdfft_tuning;
 
 /* A plan of `batch` transforms of length n. idist/odist: distance between consecutive transforms in elements of
@@ -137,6 +144,11 @@ CRDFFT_API crdfft_status crdfft_plan_create_tuned(crdfft_plan* plan, uint64_t n,
                                                   uint64_t odist, int32_t type, int32_t precision, uint32_t flags,
                                                   const crdfft_tuning* tuning);
 CRDFFT_API crdfft_status crdfft_plan_destroy(crdfft_plan plan);
+/* CRDFFT_ALGO_COOP plans: synchronizes `stream` and reports the plan's last launch on it: waits that gave up (0 when
+ * every group barrier completed, so the output is valid) and the mode (1 = groups formed, 2 = every workgroup alone,
+ * which is also valid). Other plans report 0 and 0. */
+CRDFFT_API crdfft_status crdfft_plan_coop_status(crdfft_plan plan, void* stream, uint32_t* waits_given_up,
+                                                 uint32_t* mode);
 /* BluesteinPlan<T>(IAllocator*, usize n) (bluestein.hpp:29-70): a C2C plan of any n >= 1 run as a chirp-z
  * transform over a power-of-two plan of m >= 2n-1 (crdfft_plan_info.n1 = m). Forward unnormalized; unlike FftPlan,
  * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. For
