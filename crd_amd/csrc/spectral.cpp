@@ -79,7 +79,8 @@ crdfft_status run(crdfft_spectral_plan p, const float* x, uint64_t len, int mode
                        frames,
                        p->hop,
                        mode == 0 ? 1.0f : static_cast<float>(1.0 / (fs * p->winsum2)),
-                       (p->flags & CRDFFT_SPECTRAL_DETREND) != 0 ? 1 : 0};
+                       (p->flags & CRDFFT_SPECTRAL_DETREND) != 0 ? 1 : 0,
+                       nfreq};
     if (mode != 2)
     {
         if (frames == 0)

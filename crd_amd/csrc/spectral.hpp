@@ -20,6 +20,7 @@ struct SpecArgs
     uint64_t hop;
     float scale;         // 1 / (fs * sum w^2) of the density modes
     int32_t detrend;     // subtract each frame's mean
+    uint64_t ostride;    // output row pitch in elements (complex for mode 0, float for mode 1), >= H + 1
 };
 
 // The largest H with the fused Welch lane-group mode; larger frames run density rows in chunks.

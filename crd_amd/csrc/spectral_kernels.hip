@@ -8,7 +8,7 @@
 // asked, multiplies by the window, runs the H-point Stockham core on z[j] = x[2j] + i x[2j+1], and does the
 // Hermitian recombine X[k] = E[k] + W_N^k O[k], E = (Z[k] + conj Z[H-k])/2, O = (Z[k] - conj Z[H-k])/(2i), with
 // Z[H-k] read back through LDS; X[H] = Re Z[0] - Im Z[0] comes from the k = 0 slot. The epilogue then writes
-//   MODE 0  the complex bins (STFT),
+//   MODE 0  the complex bins (STFT; with no window, no detrend and hop = the row distance, a batch of R2C rows),
 //   MODE 1  the density periodogram scale |X|^2, doubled except DC and Nyquist (spectrogram rows),
 //   MODE 2  nothing per frame: lane group g sums the periodograms of frames g, g + G, g + 2G, ... in registers and
 //           writes one partial row; k_spec_sum adds the G partial rows in order (Welch, bit-identical run to run).
@@ -133,7 +133,7 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
         }
         __syncthreads();
         // recombine and epilogue per slot (nothing but the accumulators outlives the frame)
-        const uint64_t row = f * static_cast<uint64_t>(H + 1);
+        const uint64_t row = f * a.ostride;
 #pragma unroll
         for (int m = 0; m < E; ++m)
         {

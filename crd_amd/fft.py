@@ -166,12 +166,16 @@ def describe(n, batch=1, type="c2c", precision="fp32"):
 class FftPlan:
     """A device plan of `batch` length-n transforms (contiguous at distance idist/odist, 0 = packed)."""
 
-    def __init__(self, n, batch=1, type="c2c", precision="fp32", idist=0, odist=0, self_check=True, tuning=None):
+    def __init__(self, n, batch=1, type="c2c", precision="fp32", idist=0, odist=0, self_check=True, tuning=None,
+                 passes=False):
+        """passes=True (R2C/C2R): the separate recombine pass even where the one-kernel form applies."""
         self.n = int(n)
         self.batch = int(batch)
         self.type = type
         self.handle = ctypes.c_void_p()
         flags = N.FLAG_NONE if self_check else N.FLAG_NO_SELF_CHECK
+        if passes:
+            flags |= N.FLAG_SEPARATE_PASSES
         if tuning is None:
             st = N.lib().crdfft_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist,
                                             _TYPES[type], _PRECISIONS[precision], flags)

@@ -95,8 +95,8 @@ typedef enum crdfft_algorithm
 #define CRDFFT_FLAG_NONE 0u
 #define CRDFFT_FLAG_NO_SELF_CHECK 1u       /* skip the known-answer check at creation */
 #define CRDFFT_FLAG_NORMALIZE_INVERSE 2u   /* C2C inverse scaled by 1/n (ifft_normalized, fft.hpp:1015-1029) */
-#define CRDFFT_FLAG_SEPARATE_PASSES 4u     /* Bluestein and DCT plans: the separate passes even where one kernel would
-                                              do (a differential check of the one-kernel forms) */
+#define CRDFFT_FLAG_SEPARATE_PASSES 4u     /* Bluestein, DCT, R2C and C2R plans: the separate passes even where one
+                                              kernel would do (a differential check of the one-kernel forms) */
 
 typedef struct crdfft_plan_s* crdfft_plan;
 

@@ -151,13 +151,15 @@ def test_reference_radix2_on_device(gpu, po, lg):
         assert po.rel_l2(y[0], r) <= 2 * bound(n)
 
 
-@pytest.mark.parametrize("lg", [2, 3, 5, 8, 11, 13, 15, 16, 17, 20, 21])
-def test_r2c_c2r(gpu, po, lg):
+@pytest.mark.parametrize("passes", [False, True])
+@pytest.mark.parametrize("lg", [2, 3, 5, 8, 9, 10, 11, 13, 15, 16, 17, 20, 21])
+def test_r2c_c2r(gpu, po, lg, passes):
+    """R2C/C2R against FP64 and the FP32 restatement: the one-kernel forms (n <= 32768) and the separate passes."""
     n = 1 << lg
     batch = max(1, min(9, (1 << 21) // n))
     x = po.normal_real(n * batch, lg).reshape(batch, n)
-    r2c = gpu.FftPlan(n, batch, "r2c")
-    c2r = gpu.FftPlan(n, batch, "c2r")
+    r2c = gpu.FftPlan(n, batch, "r2c", passes=passes)
+    c2r = gpu.FftPlan(n, batch, "c2r", passes=passes)
     xin = gpu.DeviceBuffer.from_array(x)
     spec = gpu.DeviceBuffer(8 * (n // 2 + 1) * batch)
     back = gpu.DeviceBuffer(4 * n * batch)
@@ -182,6 +184,42 @@ def test_r2c_c2r(gpu, po, lg):
     yb = back.download((batch, n), np.float32)
     refb = po.irfft64(Y.astype(np.complex128), n)
     assert po.rel_l2(yb, refb) <= bound(n)
+
+
+@pytest.mark.parametrize("n,batch,pad_in,pad_out", [(4, 300, 2, 3), (512, 1001, 6, 1), (1024, 301, 4, 2),
+                                                    (2048, 77, 10, 5), (32768, 9, 2, 7)])
+def test_r2c_c2r_padded_rows(gpu, po, n, batch, pad_in, pad_out):
+    """Rows idist / odist apart with gaps (many rows per workgroup, a partial last workgroup): the one-kernel forms
+    write nothing outside their rows and equal the separate passes within FP32 rounding."""
+    h = n // 2
+    ri, co = n + pad_in, h + 1 + pad_out  # R2C: reals in, complex out
+    x = po.normal_real(batch * ri, n + batch).reshape(batch, ri).astype(np.float32)
+    outs = []
+    for passes in (False, True):
+        plan = gpu.FftPlan(n, batch, "r2c", idist=ri, odist=co, passes=passes)
+        guard = np.full((batch, co), 3.0 - 1.0j, dtype=np.complex64)
+        dst = gpu.DeviceBuffer.from_array(guard)
+        plan.execute(gpu.DeviceBuffer.from_array(x), dst, gpu.FORWARD)
+        gpu.synchronize()
+        X = dst.download((batch, co), np.complex64)
+        assert np.all(X[:, h + 1:] == guard[:, h + 1:])
+        assert po.rel_l2(X[:, : h + 1], po.rfft64(x[:, :n].astype(np.float64))) <= bound(n)
+        outs.append(X[:, : h + 1])
+    assert po.rel_l2(outs[0], outs[1]) <= bound(n)
+    # C2R: complex rows in (odd pad), reals out with an even row pitch
+    ci, ro = h + 1 + pad_in, n + 2 * pad_out
+    Y = po.lcg_complex(batch * ci, n).reshape(batch, ci)
+    Y[:, 0] = Y[:, 0].real
+    Y[:, h] = Y[:, h].real
+    for passes in (False, True):
+        plan = gpu.FftPlan(n, batch, "c2r", idist=ci, odist=ro, passes=passes)
+        guard = np.full((batch, ro), 9.5, dtype=np.float32)
+        dst = gpu.DeviceBuffer.from_array(guard)
+        plan.execute(gpu.DeviceBuffer.from_array(Y), dst, gpu.INVERSE)
+        gpu.synchronize()
+        yb = dst.download((batch, ro), np.float32)
+        assert np.all(yb[:, n:] == 9.5)
+        assert po.rel_l2(yb[:, :n], po.irfft64(Y[:, : h + 1].astype(np.complex128), n)) <= bound(n)
 
 
 def test_normalized_inverse_flag(gpu, po):
