@@ -820,9 +820,10 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 // instantiation tables
 // ---------------------------------------------------------------------------------------------------------------
 #define CRDFFT_ROW_LIST(X)                                                                                             \
-    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(32, 32) X(64, 16) X(64, 64) X(128, 16) X(256, 16) X(512, 16)           \
-        X(512, 32) X(1024, 16) X(1024, 32) X(2048, 16) X(2048, 32) X(4096, 16) X(4096, 32) X(4096, 64) X(8192, 16)     \
-            X(8192, 32) X(16384, 16) X(16384, 32)
+    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(16, 4) X(32, 16) X(32, 32) X(32, 4) X(64, 16) X(64, 64) X(64, 4) X(64, 8)      \
+        X(128, 16) X(128, 4) X(128, 8) X(256, 16) X(256, 4) X(256, 8) X(512, 16) X(512, 32) X(512, 8) X(1024, 16)      \
+            X(1024, 32) X(1024, 8) X(2048, 16) X(2048, 32) X(2048, 8) X(4096, 16) X(4096, 32) X(4096, 64) X(4096, 8)   \
+                X(8192, 16) X(8192, 32) X(8192, 8) X(16384, 16) X(16384, 32)
 
 #define CRDFFT_COL_LIST(X)                                                                                             \
     X(2, 2, 64) X(4, 4, 64) X(8, 8, 64) X(16, 16, 64) X(16, 16, 16) X(32, 32, 64) X(32, 16, 16) X(64, 64, 64)          \

@@ -80,9 +80,25 @@ namespace
 // A tuned hints field: CRDFFT_HINTS_NONE asks for none, other bits are kept where the algorithm has them.
 uint32_t tuning_hints(uint32_t h, uint32_t mask) { return (h & CRDFFT_HINTS_NONE) != 0 ? 0u : (h & mask); }
 
-// Values per thread of the single kernel: every radix-16 stage but one (the first) costs one LDS round trip, and 16
-// values keep the data at 32 VGPRs; below 32 points one thread owns a whole transform.
-int default_e(uint64_t n) { return n <= 16 ? static_cast<int>(n) : 16; }
+// Values per thread of the single kernel. C2C: the fastest per length in an autotune sweep over 2^22-point batches
+// on gfx950 (fewer values per thread keep more waves resident where a transform is short, and for 1024-4096 points
+// the extra LDS round trip of radix-8 stages costs less than the occupancy radix-16 gives up). The real transforms'
+// half-length cores keep 16 (the one-kernel R2C/C2R forms are instantiated at 16): below 32 points one thread owns a
+// whole transform.
+int default_e(uint64_t n, int32_t type)
+{
+    if (type == CRDFFT_C2C)
+    {
+        switch (n)
+        {
+            case 16: case 32: case 64: return 4;
+            case 128: case 1024: case 2048: case 4096: return 8;
+            case 8192: return 32;
+            default: break;
+        }
+    }
+    return n <= 16 ? static_cast<int>(n) : 16;
+}
 
 // Four-step split of n = 2^L: n1 = 2^floor(L/2) up to 2^17, from 2^18 one bit more (measured on MI355X at 2^20:
 // 2048 x 512 ran 1.17x faster than 1024 x 1024, the row pass keeping its transposed store to 512-point rows), at most
@@ -197,7 +213,7 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
     else if (cn <= kSingleMax && (tuning == nullptr || tuning->algorithm != CRDFFT_ALGO_COOP))
     {
         s.algorithm = CRDFFT_ALGO_SINGLE;
-        s.e = default_e(cn);
+        s.e = default_e(cn, type);
         // batches of 256 points and up stream through once: non-temporal loads and stores measured 8-18% faster
         // (2^22-point batches, n = 256..16384); below that the default policy was faster
         s.hints = cn >= 256 ? kDefaultFourStepHints : 0u;

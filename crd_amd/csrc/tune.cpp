@@ -69,7 +69,7 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
     const uint64_t cn = def.complex_n;
     if (def.algorithm == CRDFFT_ALGO_SINGLE)
     {
-        for (int e : {8, 16, 32, 64})
+        for (int e : {4, 8, 16, 32, 64})
         {
             for (uint32_t hints : {CRDFFT_HINTS_NONE, 3u})
             {
