@@ -82,9 +82,8 @@ uint32_t tuning_hints(uint32_t h, uint32_t mask) { return (h & CRDFFT_HINTS_NONE
 
 // Values per thread of the single kernel. C2C: the fastest per length in an autotune sweep over 2^22-point batches
 // on gfx950 (fewer values per thread keep more waves resident where a transform is short, and for 1024-4096 points
-// the extra LDS round trip of radix-8 stages costs less than the occupancy radix-16 gives up). The real transforms'
-// half-length cores keep 16 (the one-kernel R2C/C2R forms are instantiated at 16): below 32 points one thread owns a
-// whole transform.
+// the extra LDS round trip of radix-8 stages costs less than the occupancy radix-16 gives up). R2C / C2R (n = the
+// half length here): the values the one-kernel real forms are instantiated at, so their stage table is the plan's.
 int default_e(uint64_t n, int32_t type)
 {
     if (type == CRDFFT_C2C)
@@ -97,7 +96,11 @@ int default_e(uint64_t n, int32_t type)
             default: break;
         }
     }
-    return n <= 16 ? static_cast<int>(n) : 16;
+    else if (n >= 16 && n <= 8192)
+    {
+        return n <= 256 ? 4 : 8;
+    }
+    return n <= 8 ? static_cast<int>(n) : 16;
 }
 
 // Four-step split of n = 2^L: n1 = 2^floor(L/2) up to 2^17, from 2^18 one bit more (measured on MI355X at 2^20:

@@ -135,8 +135,8 @@ __global__ __launch_bounds__((C2rCfg<H, E>::threads)) void k_c2r1(C2rArgs a)
 }
 
 #define CRDFFT_C2R1_LIST(X)                                                                                            \
-    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(512, 16) X(1024, 16) X(2048, 16)   \
-        X(4096, 16) X(8192, 16) X(16384, 16)
+    X(2, 2) X(4, 4) X(8, 8) X(16, 4) X(32, 4) X(64, 4) X(128, 4) X(256, 4) X(512, 8) X(1024, 8) X(2048, 8) X(4096, 8)  \
+        X(8192, 8) X(16384, 16)
 
 __global__ void k_fill_bytes(uint8_t* dst, uint8_t value, uint64_t bytes)
 {

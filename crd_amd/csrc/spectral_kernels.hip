@@ -243,9 +243,11 @@ __global__ __launch_bounds__(kSumBins * kSumGroups) void k_spec_sum(const float*
     }
 }
 
+// (H, E): the per-length values per thread of the real-transform cores (the same choice as the DCT kernels', whose
+// load / core / recombine structure this kernel shares)
 #define CRDFFT_SPEC_LIST(X)                                                                                            \
-    X(2, 2) X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(512, 16) X(1024, 16) X(2048, 16)   \
-        X(4096, 16) X(8192, 16) X(16384, 16)
+    X(2, 2) X(4, 4) X(8, 8) X(16, 4) X(32, 4) X(64, 4) X(128, 4) X(256, 4) X(512, 8) X(1024, 8) X(2048, 8) X(4096, 8)  \
+        X(8192, 8) X(16384, 16)
 
 template <typename K> hipError_t set_lds(K kernel, unsigned bytes)
 {
