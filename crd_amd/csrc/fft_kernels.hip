@@ -827,12 +827,14 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 
 #define CRDFFT_COL_LIST(X)                                                                                             \
     X(2, 2, 64) X(4, 4, 64) X(8, 8, 64) X(16, 16, 64) X(16, 16, 16) X(32, 32, 64) X(32, 16, 16) X(64, 64, 64)          \
-        X(64, 16, 16) X(128, 16, 16) X(128, 16, 8) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16) X(512, 16, 8)           \
-            X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+        X(64, 16, 16) X(128, 16, 16) X(128, 16, 8) X(128, 8, 16) X(256, 16, 16) X(256, 16, 8) X(256, 8, 16)            \
+            X(512, 16, 16) X(512, 16, 8) X(512, 8, 16) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8)   \
+                X(2048, 16, 4) X(4096, 16, 4)
 
 #define CRDFFT_ROWTR_LIST(X)                                                                                           \
-    X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(256, 16, 16) X(256, 16, 8) X(512, 16, 16)              \
-        X(512, 16, 8) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+    X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(128, 8, 16) X(256, 16, 16) X(256, 16, 8)               \
+        X(256, 8, 16) X(512, 16, 16) X(512, 16, 8) X(512, 8, 16) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16)        \
+            X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
 
 // (N1, E, C) of the persistent prefetching column pass
 #define CRDFFT_COL_PF_LIST(X)                                                                                          \

@@ -121,7 +121,7 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
             {
                 for (uint32_t hints : {1u, 3u})
                 {
-                    for (int variant = 0; variant < 3; ++variant)
+                    for (int variant = 0; variant < 6; ++variant)
                     {
                         crdfft_tuning t{};
                         t.algorithm = CRDFFT_ALGO_FOUR_STEP;
@@ -137,6 +137,14 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
                         {
                             t.e2 = 32;
                             t.tile2 = 16;
+                        }
+                        else if (variant >= 3)
+                        {
+                            // 8 values per thread (more waves per CU in the same LDS) in one or both passes
+                            t.e1 = variant != 4 ? 8 : 0;
+                            t.e2 = variant != 3 ? 8 : 0;
+                            t.tile1 = t.e1 != 0 ? 16 : 0;
+                            t.tile2 = t.e2 != 0 ? 16 : 0;
                         }
                         add(&t);
                     }
