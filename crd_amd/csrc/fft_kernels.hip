@@ -845,10 +845,11 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
     X(128, 16, 256, 16, 1024) X(128, 16, 256, 16, 512) X(256, 16, 256, 16, 1024) X(256, 16, 256, 16, 512)            \
         X(256, 16, 512, 16, 1024)
 
-// (m, E) of the one-kernel Bluestein transform: the default values per thread of each convolution length
+// (m, E) of the one-kernel Bluestein transform: 4 / 8 values per thread up to m = 8192 (the occupancy the two
+// m-point cores per sequence need), 16 at 16384
 #define CRDFFT_BLUE_LIST(X)                                                                                            \
-    X(4, 4) X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(512, 16) X(1024, 16) X(2048, 16)           \
-        X(4096, 16) X(8192, 16) X(16384, 16)
+    X(4, 4) X(8, 8) X(16, 4) X(32, 4) X(64, 4) X(128, 4) X(256, 4) X(512, 8) X(1024, 8) X(2048, 8) X(4096, 8)          \
+        X(8192, 8) X(16384, 16)
 
 constexpr uint64_t kMaxItems = 0xFFFFFFFFull; // work-items per launch
 
