@@ -26,6 +26,17 @@ __device__ __forceinline__ cf cmul(cf a, cf w) { return {__fmaf_rn(a.x, w.x, -a.
 // a * conj(w)
 __device__ __forceinline__ cf cmulc(cf a, cf w) { return {__fmaf_rn(a.x, w.x, a.y * w.y), __fmaf_rn(a.y, w.x, -a.x * w.y)}; }
 template <bool INV> __device__ __forceinline__ cf ctw(cf a, cf w) { return INV ? cmulc(a, w) : cmul(a, w); }
+
+// The C2R pre-rotation of one bin (real_fft.hpp:70-93): from X[k], X[h-k] and W_n^k, scale * Z[k] of the half-length
+// sequence whose inverse FFT holds the reals as z[j] = x[2j] + i x[2j+1]:
+//   Z = E + i O,  E = (X[k] + conj X[h-k]) / 2,  O = W_n^{-k} (X[k] - conj X[h-k]) / 2.
+__device__ __forceinline__ cf c2r_z(cf xk, cf xh, cf wk, float scale)
+{
+    const cf e = {(xk.x + xh.x) * 0.5f, (xk.y - xh.y) * 0.5f};
+    const cf d = {(xk.x - xh.x) * 0.5f, (xk.y + xh.y) * 0.5f};
+    const cf o = cmulc(d, wk); // W^{-k} D
+    return {(e.x - o.y) * scale, (e.y + o.x) * scale};
+}
 // a * (-i) forward, a * (+i) inverse: multiplication by W_4^1
 template <bool INV> __device__ __forceinline__ cf mul_w4(cf a) { return INV ? cf{-a.y, a.x} : cf{a.y, -a.x}; }
 

@@ -223,7 +223,10 @@ template <int N1, int E, int CC> struct ColCfg
     static constexpr int lds_bytes = (N1 == E) ? 0 : (IM + LF * CC) * 4;
 };
 
-template <int N1, int E, int CC, bool INV>
+// PRE (C2R, INV): the column values are formed from the h + 1 input bins as pre_scale * Z[j] (c2r_z) on load, which
+// replaces the separate pre-rotation pass; X[h-j] of a column lies in the mirrored column, read in the same
+// coalesced pattern backwards.
+template <int N1, int E, int CC, bool INV, bool PRE = false>
 __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a)
 {
     using C = ColCfg<N1, E, CC>;
@@ -241,8 +244,23 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
     const uint32_t off = static_cast<uint32_t>(col) + static_cast<uint32_t>(i) * ncols;
     if (active)
     {
-        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + off, C::T * ncols,
-                        (a.hints & kHintNtLoad) != 0);
+        if constexpr (PRE)
+        {
+            const cf* x = reinterpret_cast<const cf*>(a.in) + t * a.idist;
+            const cf* w = reinterpret_cast<const cf*>(a.wreal);
+            const uint32_t h = static_cast<uint32_t>(N1) * ncols;
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                const uint32_t j = off + static_cast<uint32_t>(m) * (C::T * ncols);
+                v[m] = c2r_z(x[j], x[h - j], w[j], a.pre_scale);
+            }
+        }
+        else
+        {
+            load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + off, C::T * ncols,
+                            (a.hints & kHintNtLoad) != 0);
+        }
     }
     else
     {
@@ -471,6 +489,99 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     const uint32_t n1 = static_cast<uint32_t>(a.n1);
     store_strided<E>(w, reinterpret_cast<cf*>(a.out) + t * a.odist + g * RR + r2 + static_cast<uint32_t>(j2) * n1,
                      C::T * n1, a.scale, (a.hints & kHintNtStore) != 0);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// k_rowtr_r2c: the row pass of a real transform's half-length four-step with the Hermitian recombine fused in
+// ---------------------------------------------------------------------------------------------------------------
+// The rfft of n reals is the h = n/2 point FFT Z of z[j] = x[2j] + i x[2j+1] followed by
+//   X[k] = E + W_n^k O,  E = (Z[k] + conj Z[h-k]) / 2,  O = (Z[k] - conj Z[h-k]) / (2i),  X[h] = Re Z0 - Im Z0
+// (real_fft.hpp:41-68). With h = n1 * n2 and Z[k1 + n1 k2] the row FFT of intermediate row k1, the partner of
+// (k1, k2) is (n1 - k1, n2 - 1 - k2) for k1 > 0 and (0, -k2 mod n2) for k1 = 0; row n1/2 is its own mirror. So a
+// workgroup takes Q = RR/2 rows a = pQ .. pQ+Q-1 of the lower half and their mirrors n1 - a (row n1/2 in place of
+// the mirror of row 0, which is row 0 itself): every partner is in the tile, the recombine runs on the transposed
+// LDS staging, and each X[k] is stored once, in runs of Q elements. This replaces the separate recombine pass (one
+// read and one write of the spectrum).
+template <int N2, int E, int RR>
+__global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr_r2c(RowTrArgs a)
+{
+    using C = RowTrCfg<N2, E, RR>;
+    constexpr int Q = RR / 2;
+    static_assert(RR % 2 == 0, "rows come in mirror pairs");
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int r = tid / C::T;
+    const int i = tid - r * C::T;
+    const uint64_t groups = a.n1 / RR;
+    const uint64_t lb = xcd_block();
+    const uint64_t t = lb / groups;
+    const uint64_t p = lb - t * groups;
+    const bool active = t < a.count;
+    const uint32_t n1 = static_cast<uint32_t>(a.n1);
+    // the intermediate row of tile slot q: the lower rows, then their mirrors (row n1/2 for the mirror of row 0)
+    auto row_of = [&](int q) -> uint32_t {
+        const uint32_t lo = static_cast<uint32_t>(p) * Q + static_cast<uint32_t>(q < Q ? q : q - Q);
+        return q < Q ? lo : (lo == 0 ? n1 / 2 : n1 - lo);
+    };
+    cf v[E];
+    if (active)
+    {
+        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + static_cast<uint64_t>(row_of(r)) * N2 + i,
+                        static_cast<uint32_t>(C::T), (a.hints & kHintNtLoad) != 0);
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = cf{0.0f, 0.0f};
+        }
+    }
+    const LdsView<1> view{lds + r * C::RS, C::LF};
+    fft_core<N2, E, false, 1>(v, i, view, reinterpret_cast<const cf*>(a.tw));
+    if constexpr (N2 == E)
+    {
+        __syncthreads();
+    }
+    float* tre = lds;
+    float* tim = lds + N2 * C::TP;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        const int k2 = i + C::T * m;
+        tre[k2 * C::TP + r] = v[m].x;
+        tim[k2 * C::TP + r] = v[m].y;
+    }
+    __syncthreads();
+    if (!active)
+    {
+        return;
+    }
+    const int r2 = tid % RR;
+    const int j2 = tid / RR;
+    const uint32_t k1 = row_of(r2);
+    // the partner's tile slot and how its column follows from k2
+    const bool self = (k1 == 0 || 2 * k1 == n1);
+    const int rp = self ? r2 : (r2 < Q ? r2 + Q : r2 - Q);
+    const uint64_t h = static_cast<uint64_t>(n1) * N2;
+    const cf* w = reinterpret_cast<const cf*>(a.wreal);
+    cf* out = reinterpret_cast<cf*>(a.out) + t * a.odist;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        const int k2 = j2 + C::T * m;
+        const int k2p = k1 == 0 ? ((N2 - k2) & (N2 - 1)) : (N2 - 1 - k2);
+        const cf za = cf{tre[k2 * C::TP + r2], tim[k2 * C::TP + r2]};
+        const cf zb = cf{tre[k2p * C::TP + rp], tim[k2p * C::TP + rp]};
+        const cf e = {(za.x + zb.x) * 0.5f, (za.y - zb.y) * 0.5f};
+        const cf o = {(za.y + zb.y) * 0.5f, (zb.x - za.x) * 0.5f};
+        const uint32_t k = k1 + n1 * static_cast<uint32_t>(k2);
+        out[k] = cadd(e, cmul(o, w[k]));
+        if (k == 0)
+        {
+            out[h] = cf{za.x - za.y, 0.0f};
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -836,6 +947,12 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
         X(256, 8, 16) X(512, 16, 16) X(512, 16, 8) X(512, 8, 16) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16)        \
             X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
 
+// (N1, E, C) of the C2R column pass with the pre-rotation fused in (the planner's splits of half lengths 2^15..2^24)
+#define CRDFFT_COL_C2R_LIST(X) X(128, 16, 16) X(256, 16, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
+
+// (N2, E, R) of the real-transform row pass (the four-step splits the planner gives half lengths 2^15 .. 2^24)
+#define CRDFFT_ROWTR_R2C_LIST(X) X(256, 16, 16) X(512, 8, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
+
 // (N1, E, C) of the persistent prefetching column pass
 #define CRDFFT_COL_PF_LIST(X)                                                                                          \
     X(256, 16, 16) X(512, 16, 16) X(1024, 16, 8) X(1024, 16, 16) X(1024, 32, 16) X(2048, 16, 8)
@@ -1029,6 +1146,92 @@ hipError_t launch_cols_pf(uint64_t n1, int e, int c, bool inverse, const ColArgs
         return hipGetLastError();                                                                                      \
     }
     CRDFFT_COL_PF_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+bool have_cols_c2r(uint64_t n1, int e, int c)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+        return true;
+    CRDFFT_COL_C2R_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_cols_c2r(uint64_t n1, int e, int c, const ColArgs& a, hipStream_t s)
+{
+    if (a.wreal == nullptr || a.ncols % static_cast<uint64_t>(c) != 0)
+    {
+        return hipErrorInvalidValue;
+    }
+#define X(N, E, CC)                                                                                                    \
+    if (n1 == N && e == E && c == CC)                                                                                  \
+    {                                                                                                                  \
+        using C = ColCfg<N, E, CC>;                                                                                    \
+        auto kern = k_cols<N, E, CC, true, true>;                                                                      \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t groups_per = a.ncols / CC;                                                                      \
+        const uint64_t max_t = std::max<uint64_t>(1, (kMaxItems / C::threads) / groups_per);                          \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            ColArgs part = a;                                                                                          \
+            part.count = std::min<uint64_t>(a.count - done, max_t);                                                    \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups_per * part.count)), dim3(C::threads),          \
+                               C::lds_bytes, s, part);                                                                 \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_COL_C2R_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+bool have_rowtr_r2c(uint64_t n2, int e, int r)
+{
+#define X(N, E, RR)                                                                                                    \
+    if (n2 == N && e == E && r == RR)                                                                                  \
+        return true;
+    CRDFFT_ROWTR_R2C_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_rowtr_r2c(uint64_t n2, int e, int r, const RowTrArgs& a, hipStream_t s)
+{
+    if (a.wreal == nullptr || a.n1 % static_cast<uint64_t>(r) != 0 || a.n1 < static_cast<uint64_t>(r))
+    {
+        return hipErrorInvalidValue;
+    }
+#define X(N, E, RR)                                                                                                    \
+    if (n2 == N && e == E && r == RR)                                                                                  \
+    {                                                                                                                  \
+        using C = RowTrCfg<N, E, RR>;                                                                                  \
+        auto kern = k_rowtr_r2c<N, E, RR>;                                                                             \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t groups = a.n1 / RR;                                                                             \
+        const uint64_t max_count = std::max<uint64_t>(1, 0xFFFFFFFFull / (groups * C::threads));                      \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            RowTrArgs part = a;                                                                                        \
+            part.count = std::min<uint64_t>(a.count - done, max_count);                                                \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups * part.count)), dim3(C::threads), C::lds_bytes, \
+                               s, part);                                                                               \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_ROWTR_R2C_LIST(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -54,6 +54,10 @@ struct ColArgs
     uint64_t idist;
     uint64_t odist;
     uint32_t hints;
+    // k_cols_c2r only: the input rows are the h + 1 bins of C2R transforms (h = n1 * ncols); each column value is
+    // formed as pre_scale * Z[j] from X[j], X[h-j] and wreal[j] = W_n^j on load
+    const cfloat* wreal = nullptr;
+    float pre_scale = 1.0f;
 };
 
 // Row transforms of length n2 over the n1 rows of a row-major n1 x n2 matrix, stored transposed:
@@ -69,6 +73,7 @@ struct RowTrArgs
     uint64_t odist;
     float scale; // applied to every output (1 = unscaled, exact)
     uint32_t hints;
+    const cfloat* wreal = nullptr; // k_rowtr_r2c only: W_n^k, k <= h (n = 2h), and out holds h + 1 bins per row
 };
 
 // Both four-step passes of n = n1 * n2 in one persistent launch (k_fused): workgroup b runs transforms b, b + grid,
@@ -141,10 +146,17 @@ hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStr
 hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, bool inverse, const FusedArgs& a,
                         uint64_t grid, hipStream_t s);
 hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
+// The inverse column pass of a C2R plan's half-length four-step with the pre-rotation fused into its loads.
+bool have_cols_c2r(uint64_t n1, int e, int c);
+hipError_t launch_cols_c2r(uint64_t n1, int e, int c, const ColArgs& a, hipStream_t s);
 // The persistent prefetching column pass (same arithmetic as launch_cols; ncols a multiple of c).
 bool have_cols_pf(uint64_t n1, int e, int c);
 hipError_t launch_cols_pf(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
 hipError_t launch_rowtr(uint64_t n2, int e, int r, bool inverse, const RowTrArgs& a, hipStream_t s);
+// The forward row pass of an R2C plan's half-length four-step with the Hermitian recombine fused in (k_rowtr_r2c):
+// out (a.odist complex apart) gets the h + 1 bins of each real transform; a.wreal = W_n^k, k <= h.
+bool have_rowtr_r2c(uint64_t n2, int e, int r);
+hipError_t launch_rowtr_r2c(uint64_t n2, int e, int r, const RowTrArgs& a, hipStream_t s);
 
 // Entries of the stage twiddle table of an n-point kernel at e values per thread, and the FP64 device build of it.
 uint64_t stage_twiddle_entries(uint64_t n, int e);

@@ -286,8 +286,12 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
 }
 
 // The complex transform of the plan over `count` transforms (complex distances), on `s`.
+// r2c_w (R2C plans on the four-step): the row pass also does the Hermitian recombine with this W_n^k table and
+// writes h + 1 bins per transform at odist. c2r_w (C2R plans on the four-step, inverse): the input rows are h + 1
+// bins at idist and the column pass forms (1/h) Z[j] from them on load.
 crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint64_t count, uint64_t idist,
-                          uint64_t odist, bool inverse, hipStream_t s, Profiler& prof, float scale = 1.0f)
+                          uint64_t odist, bool inverse, hipStream_t s, Profiler& prof, float scale = 1.0f,
+                          const cfloat* r2c_w = nullptr, const cfloat* c2r_w = nullptr)
 {
     const crdfft::PlanShape& sh = p->shape;
     const uint64_t cn = sh.complex_n;
@@ -331,9 +335,15 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                    cnt,
                                    idist,
                                    cn,
-                                   sh.hints & crdfft::kHintNtLoad};
+                                   sh.hints & crdfft::kHintNtLoad,
+                                   c2r_w,
+                                   1.0f / static_cast<float>(cn)};
                 CRDFFT_TRY(prof.before());
-                if ((sh.hints & crdfft::kHintPersistent) != 0)
+                if (c2r_w != nullptr)
+                {
+                    CRDFFT_TRY(crdfft::launch_cols_c2r(sh.n1, sh.e1, sh.tile1, ca, s));
+                }
+                else if ((sh.hints & crdfft::kHintPersistent) != 0)
                 {
                     CRDFFT_TRY(crdfft::launch_cols_pf(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
                 }
@@ -343,9 +353,16 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                 }
                 CRDFFT_TRY(prof.after(0, cn * cnt));
                 crdfft::RowTrArgs ra{ws, out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
-                                     cn, odist, scale, sh.hints & crdfft::kHintNtStore};
+                                     cn, odist, scale, sh.hints & crdfft::kHintNtStore, r2c_w};
                 CRDFFT_TRY(prof.before());
-                CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
+                if (r2c_w != nullptr)
+                {
+                    CRDFFT_TRY(crdfft::launch_rowtr_r2c(sh.n2, sh.e2, sh.tile2, ra, s));
+                }
+                else
+                {
+                    CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
+                }
                 CRDFFT_TRY(prof.after(1, cn * cnt));
             }
             return CRDFFT_OK;
@@ -525,6 +542,14 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
                 CRDFFT_TRY(prof.after(0, h * sh.batch));
                 return CRDFFT_OK;
             }
+            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 &&
+                (sh.hints & crdfft::kHintPersistent) == 0 && crdfft::have_rowtr_r2c(sh.n2, sh.e2, sh.tile2) &&
+                sh.n1 >= static_cast<uint64_t>(sh.tile2))
+            {
+                // n > 32768: the column pass, then the row pass with the recombine fused in (two passes, not three)
+                return run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist / 2, p->odist, false, s,
+                                   prof, 1.0f, p->wreal.as<cfloat>());
+            }
             const crdfft_status st =
                 run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist / 2, p->odist, false, s, prof);
             if (st != CRDFFT_OK)
@@ -556,6 +581,14 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
                 CRDFFT_TRY(crdfft::launch_c2r_one_kernel(h, sh.e, a, s));
                 CRDFFT_TRY(prof.after(0, h * sh.batch));
                 return CRDFFT_OK;
+            }
+            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 &&
+                (sh.hints & crdfft::kHintPersistent) == 0 && crdfft::have_cols_c2r(sh.n1, sh.e1, sh.tile1) &&
+                sh.n2 % static_cast<uint64_t>(sh.tile1) == 0)
+            {
+                // n > 32768: the pre-rotation in the column pass's loads, then the inverse row pass (two passes)
+                return run_complex(p, static_cast<const cfloat*>(in), o, sh.batch, p->idist, p->odist / 2, true, s,
+                                   prof, 1.0f, nullptr, p->wreal.as<cfloat>());
             }
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_c2r_pre(static_cast<const cfloat*>(in), o, p->wreal.as<cfloat>(), h, sh.batch,

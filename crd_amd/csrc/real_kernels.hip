@@ -56,14 +56,6 @@ __global__ void k_r2c_post(cf* data, const cf* w, uint64_t h, uint64_t count, ui
     }
 }
 
-__device__ __forceinline__ cf c2r_z(cf xk, cf xh, cf wk, float scale)
-{
-    const cf e = {(xk.x + xh.x) * 0.5f, (xk.y - xh.y) * 0.5f};
-    const cf d = {(xk.x - xh.x) * 0.5f, (xk.y + xh.y) * 0.5f};
-    const cf o = cmulc(d, wk); // W^{-k} D
-    // Z = E + i O
-    return {(e.x - o.y) * scale, (e.y + o.x) * scale};
-}
 
 __global__ void k_c2r_pre(const cf* in, cf* out, const cf* w, uint64_t h, uint64_t count, uint64_t idist,
                           uint64_t odist, float scale)
