@@ -152,6 +152,7 @@ _SIGS = {
     "crdfft_nd_execute": (_i32, [_vp, _vp, _i32, _vp]),
     "crdfft_nd_plan_destroy": (_i32, [_vp]),
     "crdfft_plan_describe": (_i32, [_u64, _u64, _i32, _i32, ctypes.POINTER(PlanInfo)]),
+    "crdfft_plan_describe_tuned": (_i32, [_u64, _u64, _i32, _i32, ctypes.POINTER(Tuning), ctypes.POINTER(PlanInfo)]),
     "crdfft_status_string": (ctypes.c_char_p, [_i32]),
     "crdfft_last_hip_error": (_i32, []),
     "crdfft_version": (ctypes.c_char_p, []),

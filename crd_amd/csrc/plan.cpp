@@ -934,12 +934,18 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
 crdfft_status crdfft_plan_describe(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
                                    crdfft_plan_info* info)
 {
+    return crdfft_plan_describe_tuned(n, batch, type, precision, nullptr, info);
+}
+
+crdfft_status crdfft_plan_describe_tuned(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
+                                         const crdfft_tuning* tuning, crdfft_plan_info* info)
+{
     if (info == nullptr)
     {
         return CRDFFT_INVALID_ARGUMENT;
     }
     crdfft::PlanShape shape;
-    const crdfft_status st = crdfft::choose_shape(n, batch, type, precision, nullptr, &shape);
+    const crdfft_status st = crdfft::choose_shape(n, batch, type, precision, tuning, &shape);
     if (st != CRDFFT_OK)
     {
         return st;

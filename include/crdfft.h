@@ -188,6 +188,10 @@ CRDFFT_API crdfft_status crdfft_plan_get_info(crdfft_plan plan, crdfft_plan_info
 /* The planner's choice for a shape without creating a plan or touching a device. */
 CRDFFT_API crdfft_status crdfft_plan_describe(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
                                               crdfft_plan_info* info);
+/* The same with tuning overrides (what crdfft_plan_create_tuned would build; the fused and cooperative forms count
+ * slots for a 256-CU device here and recount them on the plan's device). */
+CRDFFT_API crdfft_status crdfft_plan_describe_tuned(uint64_t n, uint64_t batch, int32_t type, int32_t precision,
+                                                    const crdfft_tuning* tuning, crdfft_plan_info* info);
 
 /* ---- N-dimensional transforms: NdFftPlan<T>(IAllocator*, ConstSpan<usize> dims) (nd_fft.hpp:28-196) ----
  * Row-major volumes of `ndim` dims (each >= 1), `batch` volumes back to back, in place. Forward is the unnormalized

@@ -63,6 +63,38 @@ def test_planner_real_transforms_use_half_length():
             assert crd_amd.describe(n, 2, t)["complex_n"] == n // 2
 
 
+def test_planner_defaults_follow_the_measured_tables():
+    """Values per thread and cache hints of the single kernel (C2C: the per-length sweep; R2C/C2R: the one-kernel
+    real forms' instantiations, so the plan's stage table serves them), and the 512-point row pass of the four-step."""
+    import crd_amd
+
+    want_c2c = {2: 2, 4: 4, 8: 8, 16: 4, 64: 4, 128: 8, 256: 16, 512: 16, 1024: 8, 4096: 8, 8192: 32, 16384: 16}
+    for n, e in want_c2c.items():
+        info = crd_amd.describe(n, 4)
+        assert info["e"] == e, n
+        assert info["hints"] == (3 if n >= 256 else 0), n
+    want_half = {2: 2, 4: 4, 8: 8, 16: 4, 256: 4, 512: 8, 8192: 8, 16384: 16}
+    for h, e in want_half.items():
+        for t in ("r2c", "c2r"):
+            assert crd_amd.describe(2 * h, 4, t)["e"] == e, (t, h)
+    for lg, e2 in ((17, 8), (19, 8), (20, 8), (16, 16), (21, 16)):
+        assert crd_amd.describe(1 << lg, 4)["e2"] == e2, lg
+
+
+def test_tuning_hints_none_is_explicit():
+    """CRDFFT_HINTS_NONE asks for plain loads and stores; 0 keeps the planner's default."""
+    import crd_amd
+    from crd_amd import _native as N
+
+    info = N.PlanInfo()
+    t = N.Tuning(hints=N.HINTS_NONE)
+    assert N.lib().crdfft_plan_describe_tuned(4096, 4, 0, 0, ctypes.byref(t), ctypes.byref(info)) == N.OK
+    assert info.hints == 0
+    t = N.Tuning()
+    assert N.lib().crdfft_plan_describe_tuned(4096, 4, 0, 0, ctypes.byref(t), ctypes.byref(info)) == N.OK
+    assert info.hints == crd_amd.describe(4096, 4)["hints"] == 3
+
+
 def test_planner_rejects_bad_shapes():
     import crd_amd
     from crd_amd import _native as N
