@@ -190,7 +190,7 @@ def bench_mixed(stream, Event):
         key = (n, cnt)
         if key not in plans:
             plans[key] = crd_amd.FftPlan(n, cnt, self_check=False) if n & (n - 1) == 0 else \
-                crd_amd.BluesteinPlan(n, cnt)
+                crd_amd.BluesteinPlan(n, cnt, chirp_z=True)
         calls.append((plans[key], src.address + 8 * int(offs[first]), dst.address + 8 * int(offs[first])))
 
     def plain():
@@ -244,7 +244,8 @@ def bench_spectral(stream, Event):
 def bench_r2r(stream, Event):
     """DCT-II/III and DST-II/III of 4096 rows of 4096 FP32 samples (64 MiB in, 64 MiB out): the one-kernel form and
     the separate passes around the R2C / C2R plans; GB/s counts one read and one write of every sample. Then one
-    any-length C2C, n = 1000 (Bluestein, m = 2048) over 16777 rows: the one-kernel form against the five passes."""
+    any-length C2C, n = 1000 over 16777 rows: the default (one mixed-radix launch, 1000 = 2^3 5^3), the chirp-z one-kernel
+    form (m = 2048) and its five passes."""
     import crd_amd
     from crd_amd import DeviceBuffer
     from crd_amd.dct import DctPlan
@@ -266,11 +267,10 @@ def bench_r2r(stream, Event):
     xb = np.random.default_rng(13).standard_normal((bb, 2 * bn), dtype=np.float32).view(np.complex64)
     bsrc, bdst = DeviceBuffer.from_array(xb), DeviceBuffer(xb.nbytes)
     blue = {"n": bn, "batch": bb}
-    for passes in (False, True):
-        plan = crd_amd.BluesteinPlan(bn, bb, passes=passes)
+    for name, kw in (("mixed_radix", {}), ("one_kernel", {"chirp_z": True}), ("separate_passes", {"passes": True})):
+        plan = crd_amd.BluesteinPlan(bn, bb, **kw)
         ms = time_loop(lambda: plan.execute(bsrc, bdst, crd_amd.FORWARD, stream), stream, 20, 3, Event)
-        blue["separate_passes" if passes else "one_kernel"] = {"ms": round(ms, 4),
-                                                               "gbs": round(16.0 * bn * bb / (ms * 1e-3) / 1e9, 1)}
+        blue[name] = {"ms": round(ms, 4), "gbs": round(16.0 * bn * bb / (ms * 1e-3) / 1e9, 1)}
     blue["one_kernel_vs_passes"] = round(blue["separate_passes"]["ms"] / blue["one_kernel"]["ms"], 2)
     out["bluestein"] = blue
     return out

@@ -20,6 +20,7 @@ ALGO_COPY, ALGO_SINGLE, ALGO_FOUR_STEP, ALGO_REFERENCE, ALGO_BLUESTEIN, ALGO_FUS
 HINT_COOP_SOLO = 16
 FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
 FLAG_SEPARATE_PASSES = 4
+FLAG_CHIRP_Z = 8
 HINTS_NONE = 0x100
 DCT2, DCT3, DST2, DST3 = 0, 1, 2, 3
 ND_TRANSPOSE_ALL = 1

@@ -84,6 +84,7 @@ struct crdfft_plan_s
     DeviceBuffer split_fwd, split_inv, tw256; // split-precision MFMA passes: operand splits, W_256
     // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
     crdfft_plan inner = nullptr;
+    crdfft_many_plan many = nullptr; // Bluestein plans of a 2/3/5/7-smooth n: the mixed-radix launch instead
     DeviceBuffer chirp, bhat, bwork;
     uint64_t blue_chunk = 0;
     uint64_t coop_ctl_offset = 0; // COOP: byte offset of the control words in the workspace
@@ -95,6 +96,10 @@ struct crdfft_plan_s
         if (inner != nullptr)
         {
             crdfft_plan_destroy(inner);
+        }
+        if (many != nullptr)
+        {
+            crdfft_many_plan_destroy(many);
         }
     }
 };
@@ -467,6 +472,21 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         const uint64_t n = sh.n, m = sh.n1;
         const cfloat* x = static_cast<const cfloat*>(in);
         cfloat* y = static_cast<cfloat*>(out);
+        if (p->many != nullptr)
+        {
+            CRDFFT_TRY(prof.before());
+            const crdfft_status st = crdfft_many_execute(p->many, in, out, direction, stream);
+            if (st != CRDFFT_OK)
+            {
+                return st;
+            }
+            if (inverse)
+            {
+                CRDFFT_TRY(crdfft::launch_scale(y, n * sh.batch, 1.0f / static_cast<float>(n), s)); // bluestein.hpp:86
+            }
+            CRDFFT_TRY(prof.after(0, n * sh.batch));
+            return CRDFFT_OK;
+        }
         if (p->blue_e > 0)
         {
             const crdfft::BlueArgs a{x,      y,        p->chirp.as<cfloat>(), p->bhat.as<cfloat>(), p->tw_single.as<cfloat>(),
@@ -710,6 +730,43 @@ crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64
     if (batch > 1 && (idist < n || odist < n))
     {
         return CRDFFT_INVALID_ARGUMENT;
+    }
+    // A 2/3/5/7-smooth n up to 16384 in packed rows is the same DFT as one mixed-radix launch of the persistent
+    // kernel (crdfft_many_*), about twice the chirp-z form's speed; CRDFFT_FLAG_CHIRP_Z keeps the reference's method.
+    if ((flags & (CRDFFT_FLAG_SEPARATE_PASSES | CRDFFT_FLAG_CHIRP_Z)) == 0 && n >= 2 && n <= 16384 && batch >= 1 &&
+        batch <= (uint64_t{1} << 24) && idist == n && odist == n)
+    {
+        crdfft_radix_plan cand{};
+        uint32_t ncand = 0;
+        if (crdfft_radix_candidates(n, &cand, 1, &ncand) == CRDFFT_OK && ncand > 0)
+        {
+            std::vector<uint64_t> lengths(batch, n);
+            crdfft_many_plan mp = nullptr;
+            if (crdfft_many_plan_create(&mp, lengths.data(), nullptr, batch, nullptr, 0, 0) == CRDFFT_OK)
+            {
+                auto* p = new (std::nothrow) crdfft_plan_s;
+                if (p == nullptr)
+                {
+                    crdfft_many_plan_destroy(mp);
+                    return CRDFFT_OUT_OF_MEMORY;
+                }
+                p->many = mp;
+                p->shape.n = n;
+                p->shape.batch = batch;
+                p->shape.type = CRDFFT_C2C;
+                p->shape.precision = CRDFFT_FP32;
+                p->shape.algorithm = CRDFFT_ALGO_BLUESTEIN;
+                p->shape.complex_n = n;
+                p->shape.n1 = 0; // no convolution
+                p->shape.chunk = batch;
+                p->idist = idist;
+                p->odist = odist;
+                p->flags = flags;
+                (void)hipGetDevice(&p->device);
+                *plan = p;
+                return CRDFFT_OK;
+            }
+        }
     }
     // workspace chunk: at most 256 MiB of m-point rows, at least one
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint64_t>(batch, 1),

@@ -232,13 +232,15 @@ class FftPlan:
 class BluesteinPlan(FftPlan):
     """Any-length C2C (bluestein.hpp:29-136): chirp-z over a power-of-two plan of m >= 2n-1; inverse scaled 1/n."""
 
-    def __init__(self, n, batch=1, idist=0, odist=0, passes=False):
-        """passes=True: the five separate passes even where the one-kernel form (m <= 16384) applies."""
+    def __init__(self, n, batch=1, idist=0, odist=0, passes=False, chirp_z=False):
+        """passes=True: the five separate chirp-z passes even where the one-kernel form (m <= 16384) applies;
+        chirp_z=True: the chirp-z algorithm even for the 2/3/5/7-smooth n that otherwise run as one mixed-radix
+        launch."""
         self.n = int(n)
         self.batch = int(batch)
         self.type = "c2c"
         self.handle = ctypes.c_void_p()
-        flags = N.FLAG_SEPARATE_PASSES if passes else 0
+        flags = (N.FLAG_SEPARATE_PASSES if passes else 0) | (N.FLAG_CHIRP_Z if chirp_z else 0)
         check(N.lib().crdfft_bluestein_plan_create(ctypes.byref(self.handle), self.n, self.batch, idist, odist, flags),
               f"crdfft_bluestein_plan_create(n={n}, batch={batch})")
 

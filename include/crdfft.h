@@ -97,6 +97,8 @@ typedef enum crdfft_algorithm
 #define CRDFFT_FLAG_NORMALIZE_INVERSE 2u   /* C2C inverse scaled by 1/n (ifft_normalized, fft.hpp:1015-1029) */
 #define CRDFFT_FLAG_SEPARATE_PASSES 4u     /* Bluestein, DCT, R2C and C2R plans: the separate passes even where one
                                               kernel would do (a differential check of the one-kernel forms) */
+#define CRDFFT_FLAG_CHIRP_Z 8u             /* Bluestein plans: the chirp-z algorithm even for the 2/3/5/7-smooth n
+                                              that otherwise run as one mixed-radix launch */
 
 typedef struct crdfft_plan_s* crdfft_plan;
 
@@ -151,9 +153,11 @@ CRDFFT_API crdfft_status crdfft_plan_coop_status(crdfft_plan plan, void* stream,
                                                  uint32_t* mode);
 /* BluesteinPlan<T>(IAllocator*, usize n) (bluestein.hpp:29-70): a C2C plan of any n >= 1 run as a chirp-z
  * transform over a power-of-two plan of m >= 2n-1 (crdfft_plan_info.n1 = m). Forward unnormalized; unlike FftPlan,
- * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. For
- * m <= 16384 (n <= 8192) one kernel runs the whole transform per sequence (x read once, X written once, no
- * workspace: crdfft_plan_info.e > 0, workspace_bytes 0); longer n run the five passes over an m-point workspace. */
+ * the inverse is scaled by 1/n (bluestein.hpp:86). Executed with crdfft_execute; in == out is allowed. A
+ * 2/3/5/7-smooth n <= 16384 in packed rows runs as one mixed-radix launch of the persistent kernel (crdfft_many_*;
+ * crdfft_plan_info.n1 = 0: no convolution) unless CRDFFT_FLAG_CHIRP_Z. Otherwise, for m <= 16384 (n <= 8192) one
+ * kernel runs the whole chirp-z transform per sequence (x read once, X written once, no workspace:
+ * crdfft_plan_info.e > 0, workspace_bytes 0); longer n run the five passes over an m-point workspace. */
 CRDFFT_API crdfft_status crdfft_bluestein_plan_create(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_t idist,
                                                       uint64_t odist, uint32_t flags);
 

@@ -306,18 +306,30 @@ def test_cpp_interface_runs(gpu, tmp_path):
     assert r.returncode == 0
 
 
-@pytest.mark.parametrize("passes", [False, True])
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 8191, 8192, 10007, 7 * 11 * 13 * 17])
-def test_bluestein_any_length(gpu, po, n, passes):
+def smooth7(n):
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+@pytest.mark.parametrize("mode", ["default", "chirp_z", "passes"])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 8191, 8192, 10007, 7 * 11 * 13 * 17,
+                               3 * 5 * 7 * 11 * 13])
+def test_bluestein_any_length(gpu, po, n, mode):
     """BluesteinPlan (bluestein.hpp): forward unnormalized, inverse normalized, vs FP64 and the FP32 restatement;
-    the one-kernel form (m <= 16384) and the five separate passes."""
+    2/3/5/7-smooth n as one mixed-radix launch, the chirp-z one-kernel form (m <= 16384) and its five passes."""
     batch = 3
     x = po.lcg_complex(n * batch, 7 * n + 1).reshape(batch, n)
-    plan = gpu.BluesteinPlan(n, batch, passes=passes)
-    assert plan.conv_size() >= 2 * n - 1
+    plan = gpu.BluesteinPlan(n, batch, passes=mode == "passes", chirp_z=mode == "chirp_z")
+    direct = mode == "default" and 2 <= n <= 16384 and smooth7(n)
     info = plan.info()
-    one_kernel = not passes and 4 <= plan.conv_size() <= 16384
-    assert (info["e"] > 0) == one_kernel and (info["workspace_bytes"] == 0) == one_kernel
+    if direct:
+        assert plan.conv_size() == 0 and info["workspace_bytes"] == 0
+    else:
+        assert plan.conv_size() >= 2 * n - 1
+        one_kernel = mode != "passes" and 4 <= plan.conv_size() <= 16384
+        assert (info["e"] > 0) == one_kernel and (info["workspace_bytes"] == 0) == one_kernel
     src = gpu.DeviceBuffer.from_array(x)
     dst = gpu.DeviceBuffer(x.nbytes)
     for inverse in (False, True):
@@ -348,7 +360,7 @@ def test_bluestein_one_kernel_batches(gpu, po, n, batch, idist, odist):
     guard = np.full((batch, odist), 7.0 + 3.0j, dtype=np.complex64)
     outs = []
     for passes in (False, True):
-        plan = gpu.BluesteinPlan(n, batch, idist, odist, passes=passes)
+        plan = gpu.BluesteinPlan(n, batch, idist, odist, passes=passes, chirp_z=True)
         dst = gpu.DeviceBuffer.from_array(guard)
         for inverse in (False, True):
             plan.execute(src, dst, gpu.INVERSE if inverse else gpu.FORWARD)
