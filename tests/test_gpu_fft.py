@@ -187,7 +187,8 @@ def test_r2c_c2r(gpu, po, lg, passes):
 
 
 @pytest.mark.parametrize("n,batch,pad_in,pad_out", [(4, 300, 2, 3), (512, 1001, 6, 1), (1024, 301, 4, 2),
-                                                    (2048, 77, 10, 5), (32768, 9, 2, 7)])
+                                                    (2048, 77, 10, 5), (32768, 9, 2, 7), (65536, 5, 2, 3),
+                                                    (1 << 18, 3, 4, 1)])
 def test_r2c_c2r_padded_rows(gpu, po, n, batch, pad_in, pad_out):
     """Rows idist / odist apart with gaps (many rows per workgroup, a partial last workgroup): the one-kernel forms
     write nothing outside their rows and equal the separate passes within FP32 rounding."""
@@ -220,6 +221,28 @@ def test_r2c_c2r_padded_rows(gpu, po, n, batch, pad_in, pad_out):
         yb = dst.download((batch, ro), np.float32)
         assert np.all(yb[:, n:] == 9.5)
         assert po.rel_l2(yb[:, :n], po.irfft64(Y[:, : h + 1].astype(np.complex128), n)) <= bound(n)
+
+
+def test_real_four_step_over_several_chunks(gpu, po):
+    """n = 2^16 real (half length 32768 on the four-step, 1024 transforms per 256 MiB chunk) over 1030 rows: the
+    fused row pass (R2C) and fused column pass (C2R) across a chunk boundary, checked on rows either side of it."""
+    n, batch = 1 << 16, 1030
+    h = n // 2
+    x = po.normal_real(n * batch, 77).reshape(batch, n).astype(np.float32)
+    r2c = gpu.FftPlan(n, batch, "r2c")
+    c2r = gpu.FftPlan(n, batch, "c2r")
+    assert r2c.info()["algorithm"] == 2 and r2c.info()["chunk"] < batch
+    xin = gpu.DeviceBuffer.from_array(x)
+    spec = gpu.DeviceBuffer(8 * (h + 1) * batch)
+    back = gpu.DeviceBuffer(4 * n * batch)
+    r2c.execute(xin, spec, gpu.FORWARD)
+    c2r.execute(spec, back, gpu.INVERSE)
+    gpu.synchronize()
+    rows = [0, 1, 1022, 1023, 1024, 1025, batch - 1]
+    X = spec.download((batch, h + 1), np.complex64)[rows]
+    assert po.rel_l2(X, po.rfft64(x[rows].astype(np.float64))) <= bound(n)
+    xb = back.download((batch, n), np.float32)[rows]
+    assert np.max(np.abs(xb - x[rows])) <= bound(n) * np.max(np.abs(x[rows]))
 
 
 def test_normalized_inverse_flag(gpu, po):
