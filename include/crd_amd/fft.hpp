@@ -1,6 +1,7 @@
-// C++20 host interface of the MI355X batched FFT path: the reference's FftPlan / RealFftPlan / fft /
-// ifft_normalized (engine/numerics/hesap-fft/include/crd/hesap/fft/fft.hpp:220-1029, real_fft.hpp:18-101) over the
-// crdfft C-ABI, header-only.
+// C++20 host interface of the MI355X batched FFT path: the reference's FftPlan / RealFftPlan / BluesteinPlan /
+// NdFftPlan / DctPlan / fft / ifft_normalized (engine/numerics/hesap-fft/include/crd/hesap/fft/fft.hpp:220-1029,
+// real_fft.hpp:18-101, bluestein.hpp, nd_fft.hpp, dct.hpp) and hesap-dsp's spectral front end over the crdfft C-ABI,
+// header-only.
 //
 // Same names, argument meaning and conventions as the reference: forward and inverse unnormalized, irfft scaled by
 // 1/n, Complex<T> interleaved {re, im}, execute in place on n values, execute_batched element-major
@@ -237,6 +238,171 @@ private:
     void* m_stream;
     crdfft_plan m_r2c = nullptr;
     crdfft_plan m_c2r = nullptr;
+};
+
+namespace detail
+{
+// An owned crdfft handle of one of the plan kinds (the reference's plans are pinned, so are these).
+template <typename H, crdfft_status (*Destroy)(H)> class Handle
+{
+public:
+    Handle() = default;
+    ~Handle()
+    {
+        if (m_h != nullptr)
+        {
+            (void)Destroy(m_h);
+        }
+    }
+    Handle(const Handle&) = delete;
+    Handle& operator=(const Handle&) = delete;
+    H* out() noexcept { return &m_h; }
+    [[nodiscard]] H get() const noexcept { return m_h; }
+
+private:
+    H m_h = nullptr;
+};
+} // namespace detail
+
+// Any-length C2C (bluestein.hpp:29-136): forward unnormalized, inverse scaled by 1/n; `batch` contiguous transforms.
+template <typename T> class BluesteinPlan
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+
+public:
+    explicit BluesteinPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr)
+        : m_n(n), m_batch(batch), m_stream(stream)
+    {
+        check(crdfft_bluestein_plan_create(m_plan.out(), n, batch, 0, 0, CRDFFT_FLAG_NONE),
+              "crdfft_bluestein_plan_create");
+    }
+    [[nodiscard]] std::size_t size() const noexcept { return m_n; }
+    void execute(DeviceSpan<Complex<T>> data, FftDirection dir) const
+    {
+        if (data.size() != m_n * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "BluesteinPlan::execute: size mismatch");
+        }
+        check(crdfft_execute(m_plan.get(), data.data(), data.data(), static_cast<int32_t>(dir), m_stream),
+              "crdfft_execute(Bluestein)");
+    }
+
+private:
+    std::size_t m_n;
+    std::size_t m_batch;
+    void* m_stream;
+    detail::Handle<crdfft_plan, crdfft_plan_destroy> m_plan;
+};
+
+// Row-major N-D C2C of `batch` volumes in place (nd_fft.hpp:28-196): inverse scaled by 1/prod(dims).
+template <typename T> class NdFftPlan
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+
+public:
+    NdFftPlan(const std::size_t* dims, std::uint32_t ndim, std::size_t batch = 1, void* stream = nullptr)
+        : m_total(batch), m_stream(stream)
+    {
+        std::uint64_t d[8] = {};
+        if (ndim == 0 || ndim > 8)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "NdFftPlan: 1..8 dimensions");
+        }
+        for (std::uint32_t i = 0; i < ndim; ++i)
+        {
+            d[i] = dims[i];
+            m_total *= dims[i];
+        }
+        check(crdfft_nd_plan_create(m_plan.out(), d, ndim, batch, 0), "crdfft_nd_plan_create");
+    }
+    void execute(DeviceSpan<Complex<T>> data, FftDirection dir) const
+    {
+        if (data.size() != m_total)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "NdFftPlan::execute: size mismatch");
+        }
+        check(crdfft_nd_execute(m_plan.get(), data.data(), static_cast<int32_t>(dir), m_stream), "crdfft_nd_execute");
+    }
+
+private:
+    std::size_t m_total;
+    void* m_stream;
+    detail::Handle<crdfft_nd_plan, crdfft_nd_plan_destroy> m_plan;
+};
+
+// DCT-II/III and DST-II/III of `batch` rows of n reals (dct.hpp:34-170, scipy norm=None conventions).
+template <typename T> class DctPlan
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+
+public:
+    explicit DctPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr)
+        : m_n(n), m_batch(batch), m_stream(stream)
+    {
+        check(crdfft_dct_plan_create(m_plan.out(), n, batch, 0, 0, CRDFFT_FLAG_NONE), "crdfft_dct_plan_create");
+    }
+    [[nodiscard]] std::size_t size() const noexcept { return m_n; }
+    void dct2(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DCT2, x, y); }
+    void dct3(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DCT3, x, y); }
+    void dst2(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DST2, x, y); }
+    void dst3(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DST3, x, y); }
+
+private:
+    void run(crdfft_r2r_kind kind, DeviceSpan<const T> x, DeviceSpan<T> y) const
+    {
+        if (x.size() != m_n * m_batch || y.size() != m_n * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "DctPlan: size mismatch");
+        }
+        check(crdfft_dct_execute(m_plan.get(), kind, x.data(), y.data(), m_stream), "crdfft_dct_execute");
+    }
+    std::size_t m_n;
+    std::size_t m_batch;
+    void* m_stream;
+    detail::Handle<crdfft_dct_plan, crdfft_dct_plan_destroy> m_plan;
+};
+
+// hesap-dsp's STFT / spectrogram / Welch over frames of nperseg samples at hop nperseg - noverlap
+// (spectral.hpp:43-222, :334-386), each one fused launch.
+class SpectralPlan
+{
+public:
+    SpectralPlan(std::size_t nperseg, std::size_t noverlap, crdfft_window window, bool detrend,
+                 void* stream = nullptr)
+        : m_nfreq(nperseg / 2 + 1), m_stream(stream)
+    {
+        check(crdfft_spectral_plan_create(m_plan.out(), nperseg, noverlap, window,
+                                          detrend ? CRDFFT_SPECTRAL_DETREND : 0u),
+              "crdfft_spectral_plan_create");
+    }
+    [[nodiscard]] std::size_t frames(std::size_t len) const { return crdfft_spectral_frames(m_plan.get(), len); }
+    void stft(DeviceSpan<const float> x, DeviceSpan<Complex<float>> out) const
+    {
+        need(out.size() == frames(x.size()) * m_nfreq, "SpectralPlan::stft: size mismatch");
+        check(crdfft_stft(m_plan.get(), x.data(), x.size(), out.data(), m_stream), "crdfft_stft");
+    }
+    void spectrogram(DeviceSpan<const float> x, double fs, DeviceSpan<float> out) const
+    {
+        need(out.size() == frames(x.size()) * m_nfreq, "SpectralPlan::spectrogram: size mismatch");
+        check(crdfft_spectrogram(m_plan.get(), x.data(), x.size(), fs, out.data(), m_stream), "crdfft_spectrogram");
+    }
+    void welch(DeviceSpan<const float> x, double fs, DeviceSpan<float> psd) const
+    {
+        need(psd.size() == m_nfreq, "SpectralPlan::welch: size mismatch");
+        check(crdfft_welch(m_plan.get(), x.data(), x.size(), fs, psd.data(), m_stream), "crdfft_welch");
+    }
+
+private:
+    static void need(bool ok, const char* what)
+    {
+        if (!ok)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, what);
+        }
+    }
+    std::size_t m_nfreq;
+    void* m_stream;
+    detail::Handle<crdfft_spectral_plan, crdfft_spectral_plan_destroy> m_plan;
 };
 
 // One-shot transient plan, in place, unnormalized (fft.hpp:1005-1012). Synchronous.
