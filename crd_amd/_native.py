@@ -141,6 +141,10 @@ _SIGS = {
     "crdfft_plan_create_tuned": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _i32, _i32, _u32,
                                         ctypes.POINTER(Tuning)]),
     "crdfft_plan_destroy": (_i32, [_vp]),
+    "crdfft_graph_begin": (_i32, [_vp]),
+    "crdfft_graph_end": (_i32, [_vp, ctypes.POINTER(_vp)]),
+    "crdfft_graph_launch": (_i32, [_vp, _vp]),
+    "crdfft_graph_destroy": (_i32, [_vp]),
     "crdfft_plan_coop_status": (_i32, [_vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "crdfft_bluestein_plan_create": (_i32, [ctypes.POINTER(_vp), _u64, _u64, _u64, _u64, _u32]),
     "crdfft_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),

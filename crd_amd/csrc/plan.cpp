@@ -1200,4 +1200,51 @@ crdfft_status crdfft_event_elapsed_ms(float* ms, void* start, void* stop)
     return CRDFFT_OK;
 }
 
+crdfft_status crdfft_graph_begin(void* stream)
+{
+    if (stream == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT; // the null stream cannot be captured
+    }
+    CRDFFT_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_graph_end(void* stream, void** graph)
+{
+    if (stream == nullptr || graph == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    CRDFFT_TRY(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CRDFFT_TRY(e);
+    *graph = exec;
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_graph_launch(void* graph, void* stream)
+{
+    if (graph == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    CRDFFT_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), static_cast<hipStream_t>(stream)));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_graph_destroy(void* graph)
+{
+    if (graph == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    CRDFFT_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+    return CRDFFT_OK;
+}
+
 } // extern "C"

@@ -51,6 +51,17 @@ CRDFFT_API crdfft_status crdfft_event_record(void* event, void* stream);
 CRDFFT_API crdfft_status crdfft_event_synchronize(void* event);
 CRDFFT_API crdfft_status crdfft_event_elapsed_ms(float* ms, void* start, void* stop);
 
+/* hipGraph capture of any sequence of executes on one (non-null) stream: crdfft_graph_begin starts capturing,
+ * crdfft_graph_end stops and instantiates it (*graph), crdfft_graph_launch replays it on any stream. Every plan's
+ * execute is capturable once the plan has executed on that stream before (its per-stream scratch then exists; a
+ * first execute on a new stream allocates and must not be captured). The plans record no memset nodes (the
+ * cooperative plan clears its control words with a kernel); the Welch PSD of a signal shorter than one frame is the
+ * one call that records a memset, and is not meant for capture. */
+CRDFFT_API crdfft_status crdfft_graph_begin(void* stream);
+CRDFFT_API crdfft_status crdfft_graph_end(void* stream, void** graph);
+CRDFFT_API crdfft_status crdfft_graph_launch(void* graph, void* stream);
+CRDFFT_API crdfft_status crdfft_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif
