@@ -1,6 +1,6 @@
 """The learned plan cost model: a 3-layer MLP (32 plan features -> 64 -> 64 -> 1, FP32) that predicts log(time) of a
 plan, trained and evaluated in libcrdfft's own HIP kernels, used to rank autotuner candidates so only the top k are
-timed. Exhaustive timing (crdfft_autotune_plan) stays the fallback and the judge of the ranking.
+timed. Exhaustive timing (crdfft_autotune_plan) stays the fallback and the measure the ranking is scored against.
 
 The reference has nothing learned: its planner is a checked-in table measured offline
 (engine/numerics/hesap-fft/include/crd/hesap/fft/detail/level_plans_gen.hpp, scripts/gen_fft_level_plans.py).

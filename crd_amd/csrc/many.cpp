@@ -671,7 +671,7 @@ crdfft_status crdfft_many_autotune(const uint64_t* lengths, uint64_t count, uint
         }
         choices[i] = best_plan;
     }
-    // the per-length winners are measured alone; the batch as given is the judge: if the planner's defaults run it
+    // the per-length winners are measured alone; the batch as given decides: if the planner's defaults run it
     // faster, they are returned instead (skipped when the batch does not fit beside the tuning buffers)
     if (result == CRDFFT_OK && count > 0)
     {
