@@ -178,10 +178,13 @@ __device__ __forceinline__ void many_stage(const TileCtx& c, const ManyStage& st
             const uint32_t j = mdiv(g, st.nr_magic);
             const uint32_t b = g - j * nr;
             const uint32_t k = b - mdiv(b, st.p_magic) * st.p;
-            if (st.p > 1)
+            if (R > 1 && st.p > 1) // a radix-1 stage (n = 1) has no twiddle
             {
                 const uint32_t e = k * st.twstep; // W_{pR}^{k} = W_n^{e}
-                if constexpr (R <= 8 || R == 16)
+                if constexpr (R == 1)
+                {
+                }
+                else if constexpr (R <= 8 || R == 16)
                 {
                     cf w[R];
                     w[1] = cmul(c.thi[e >> 7], c.tlo[e & (kManyTwLo - 1)]);
