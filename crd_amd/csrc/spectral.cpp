@@ -12,6 +12,7 @@
 #include "kernels.hpp"
 #include "planner.hpp"
 #include "scratch.hpp"
+#include "real.hpp"
 #include "spectral.hpp"
 
 namespace
@@ -93,7 +94,7 @@ crdfft_status run(crdfft_spectral_plan p, const float* x, uint64_t len, int mode
     float* psd = static_cast<float*>(out);
     if (frames == 0)
     {
-        SPEC_TRY(hipMemsetAsync(psd, 0, nfreq * sizeof(float), s));
+        SPEC_TRY(crdfft::launch_fill_bytes(psd, 0, nfreq * sizeof(float), s)); // a kernel: safe in a captured graph
         return CRDFFT_OK;
     }
     float* partial = static_cast<float*>(p->partial_of.get(p->partial, s));

@@ -297,6 +297,42 @@ def test_error_paths(gpu):
     assert e.value.status == N.INVALID_ARGUMENT
 
 
+def test_empty_batches_are_no_ops(gpu):
+    """Zero transforms (the reference's run_batch over an empty range, batch_arms.hpp:46-77; a signal shorter than
+    one frame, spectral.hpp:43-126): every execute returns OK and leaves the buffers as they were, except Welch,
+    whose PSD of no frames is all zeros."""
+    from crd_amd import _native as N
+    from crd_amd import spectral
+    from crd_amd.dct import DctPlan
+
+    guard = np.full(1024, 0x7FC00001, dtype=np.uint32)  # a quiet-NaN pattern
+    a, b = gpu.DeviceBuffer.from_array(guard), gpu.DeviceBuffer.from_array(guard)
+    plans = [gpu.FftPlan(4096, 0), gpu.FftPlan(1 << 16, 0), gpu.FftPlan(1 << 20, 0), gpu.FftPlan(4096, 0, "r2c"),
+             gpu.FftPlan(1 << 18, 0, "r2c"), gpu.FftPlan(4096, 0, "c2r"), gpu.FftPlan(1 << 18, 0, "c2r"),
+             gpu.BluesteinPlan(1000, 0), gpu.BluesteinPlan(1009, 0), gpu.BluesteinPlan(1000, 0, passes=True)]
+    for p in plans:
+        t = p.info()["type"]
+        dirs = (gpu.FORWARD, gpu.INVERSE) if t == N.C2C else ((gpu.FORWARD,) if t == N.R2C else (gpu.INVERSE,))
+        for d in dirs:
+            p.execute(a, b, d)
+    gpu.FftPlan(4096, 8).execute_batched(a, 0)
+    for passes in (False, True):
+        dp = DctPlan(4096, 0, passes=passes)
+        for kind in range(4):
+            dp.execute(kind, a, b)
+    sp = spectral.SpectralPlan(256, 128)
+    assert sp.frames(100) == 0
+    sp.stft(a, 100, b)
+    sp.spectrogram(a, 100, 1.0, b)
+    gpu.synchronize()
+    assert np.array_equal(a.download(guard.shape, np.uint32), guard)
+    assert np.array_equal(b.download(guard.shape, np.uint32), guard)
+    sp.welch(a, 100, 1.0, b)
+    gpu.synchronize()
+    psd = b.download(guard.shape, np.uint32)
+    assert not psd[:129].any() and np.array_equal(psd[129:], guard[129:])
+
+
 def test_self_check_fault_seam(gpu):
     """CRDFFT_FAULT_SELF_CHECK=1 flips an output bit on the host: plan creation must fail with SELF_CHECK_FAILED
     (the reference forces its demotion path the same way, fault_seam.hpp:139)."""
