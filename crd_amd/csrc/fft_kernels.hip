@@ -249,11 +249,15 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
             const cf* x = reinterpret_cast<const cf*>(a.in) + t * a.idist;
             const cf* w = reinterpret_cast<const cf*>(a.wreal);
             const uint32_t h = static_cast<uint32_t>(N1) * ncols;
+            // W_n^j for j = col + ncols (i + T m) as W_n^{col} W_n^{ncols i} W_n^{ncols T m}: two table reads per
+            // thread and one workgroup-uniform read per value instead of a gather of the h-entry table
+            const cf wb = cmul(w[col], w[static_cast<uint32_t>(i) * ncols]);
 #pragma unroll
             for (int m = 0; m < E; ++m)
             {
                 const uint32_t j = off + static_cast<uint32_t>(m) * (C::T * ncols);
-                v[m] = c2r_z(x[j], x[h - j], w[j], a.pre_scale);
+                const cf wj = m == 0 ? wb : cmul(wb, w[static_cast<uint32_t>(m) * (C::T * ncols)]);
+                v[m] = c2r_z(x[j], x[h - j], wj, a.pre_scale);
             }
         }
         else
@@ -526,8 +530,11 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr_r2c(Ro
     cf v[E];
     if (active)
     {
+        // plain loads and stores whatever the plan's hints: the intermediate is read back from the caches, and the
+        // output goes out in runs of Q elements, which non-temporal stores write as partial lines (measured 1.6x
+        // slower at n = 2^16)
         load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + static_cast<uint64_t>(row_of(r)) * N2 + i,
-                        static_cast<uint32_t>(C::T), (a.hints & kHintNtLoad) != 0);
+                        static_cast<uint32_t>(C::T), false);
     }
     else
     {
@@ -564,8 +571,11 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr_r2c(Ro
     const bool self = (k1 == 0 || 2 * k1 == n1);
     const int rp = self ? r2 : (r2 < Q ? r2 + Q : r2 - Q);
     const uint64_t h = static_cast<uint64_t>(n1) * N2;
+    // W_n^k for k = k1 + n1 (j2 + T m) as W_n^{k1} W_n^{n1 j2} W_n^{n1 T m}: two table reads per thread and one
+    // workgroup-uniform (scalar) read per value, instead of a gather of the h-entry table per output
     const cf* w = reinterpret_cast<const cf*>(a.wreal);
-    cf* out = reinterpret_cast<cf*>(a.out) + t * a.odist;
+    const cf wb = cmul(w[k1], w[n1 * static_cast<uint32_t>(j2)]);
+    const __amdgpu_buffer_rsrc_t out = make_rsrc(reinterpret_cast<cf*>(a.out) + t * a.odist);
 #pragma unroll
     for (int m = 0; m < E; ++m)
     {
@@ -576,10 +586,11 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr_r2c(Ro
         const cf e = {(za.x + zb.x) * 0.5f, (za.y - zb.y) * 0.5f};
         const cf o = {(za.y + zb.y) * 0.5f, (zb.x - za.x) * 0.5f};
         const uint32_t k = k1 + n1 * static_cast<uint32_t>(k2);
-        out[k] = cadd(e, cmul(o, w[k]));
+        const cf wk = m == 0 ? wb : cmul(wb, w[n1 * static_cast<uint32_t>(C::T * m)]);
+        bst<0>(out, k * 8u, 0, cadd(e, cmul(o, wk)));
         if (k == 0)
         {
-            out[h] = cf{za.x - za.y, 0.0f};
+            bst<0>(out, static_cast<uint32_t>(h) * 8u, 0, cf{za.x - za.y, 0.0f});
         }
     }
 }
