@@ -588,7 +588,7 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr_r2c(Ro
         const uint32_t k = k1 + n1 * static_cast<uint32_t>(k2);
         const cf wk = m == 0 ? wb : cmul(wb, w[n1 * static_cast<uint32_t>(C::T * m)]);
         bst<0>(out, k * 8u, 0, cadd(e, cmul(o, wk)));
-        if (k == 0)
+        if (m == 0 && k == 0) // k >= n1 * T for m >= 1
         {
             bst<0>(out, static_cast<uint32_t>(h) * 8u, 0, cf{za.x - za.y, 0.0f});
         }
@@ -955,14 +955,15 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 
 #define CRDFFT_ROWTR_LIST(X)                                                                                           \
     X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(128, 8, 16) X(256, 16, 16) X(256, 16, 8)               \
-        X(256, 8, 16) X(512, 16, 16) X(512, 16, 8) X(512, 8, 16) X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16)        \
-            X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
+        X(256, 8, 16) X(256, 16, 32) X(256, 8, 32) X(512, 16, 16) X(512, 16, 8) X(512, 8, 16) X(1024, 16, 16)          \
+            X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
 
 // (N1, E, C) of the C2R column pass with the pre-rotation fused in (the planner's splits of half lengths 2^15..2^24)
 #define CRDFFT_COL_C2R_LIST(X) X(128, 16, 16) X(256, 16, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
 
 // (N2, E, R) of the real-transform row pass (the four-step splits the planner gives half lengths 2^15 .. 2^24)
-#define CRDFFT_ROWTR_R2C_LIST(X) X(256, 16, 16) X(512, 8, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
+#define CRDFFT_ROWTR_R2C_LIST(X)                                                                                       \
+    X(256, 16, 16) X(256, 16, 32) X(256, 8, 32) X(512, 8, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
 
 // (N1, E, C) of the persistent prefetching column pass
 #define CRDFFT_COL_PF_LIST(X)                                                                                          \
