@@ -122,6 +122,12 @@ def test_null_handles_are_rejected_without_a_device():
     assert L.crdfft_execute_batched(None, None, 1, 0, None) == N.INVALID_ARGUMENT
     assert L.crdfft_self_check(None) == N.INVALID_ARGUMENT
     assert L.crdfft_plan_create(None, 64, 1, 0, 0, 0, 0, 0) == N.INVALID_ARGUMENT
+    # graph capture: the null stream cannot be captured, and null graphs are refused before any HIP call
+    g = ctypes.c_void_p()
+    assert L.crdfft_graph_begin(None) == N.INVALID_ARGUMENT
+    assert L.crdfft_graph_end(None, ctypes.byref(g)) == N.INVALID_ARGUMENT
+    assert L.crdfft_graph_launch(None, None) == N.INVALID_ARGUMENT
+    assert L.crdfft_graph_destroy(None) == N.INVALID_ARGUMENT
 
 
 def test_cpp_interface_compiles():
