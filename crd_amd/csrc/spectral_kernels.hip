@@ -66,6 +66,8 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
     // MODE 0/1: one frame per lane group; MODE 2: frames group, group + groups, ... (uniform trip count per
     // workgroup: every lane group runs the same number of iterations, idle ones on a masked frame)
     const uint64_t iters = MODE == 2 ? (a.frames + groups - 1) / groups : 1;
+    // 8-byte sample pairs when every frame starts 8-byte aligned (an even hop from an 8-byte aligned signal)
+    const bool pairs = (reinterpret_cast<uintptr_t>(a.x) & 7u) == 0 && (a.hop & 1u) == 0;
 #pragma unroll 1
     for (uint64_t it = 0; it < iters; ++it)
     {
@@ -81,30 +83,57 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
         // no memory
         const __amdgpu_buffer_rsrc_t xr = make_rsrc(a.x + (active ? f * a.hop : 0), active ? 8u * H : 0u);
         cf v[E];
-        float part = 0.0f;
-#pragma unroll
-        for (int m = 0; m < E; ++m)
+        if (pairs)
         {
-            const uint32_t j = static_cast<uint32_t>(i + T * m);
-            v[m] = cf{bld32<0>(xr, 8u * j, 0), bld32<0>(xr, 8u * j + 4u, 0)};
-            part += v[m].x + v[m].y;
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                v[m] = bld<0>(xr, 8u * static_cast<uint32_t>(i + T * m), 0);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+            {
+                const uint32_t j = static_cast<uint32_t>(i + T * m);
+                v[m] = cf{bld32<0>(xr, 8u * j, 0), bld32<0>(xr, 8u * j + 4u, 0)};
+            }
         }
         if (a.detrend != 0)
         {
-            // the frame mean: tree sum over the frame's T threads, fixed order
-            red[tid] = part;
-            __syncthreads();
+            // the frame mean, in a fixed order: a butterfly over the frame's lanes of one wave (each step adds the
+            // same two partial sums in both lanes, so every lane ends with the same bits), then the waves' sums in
+            // wave order through LDS
+            float part = 0.0f;
 #pragma unroll
-            for (int s = T / 2; s > 0; s >>= 1)
+            for (int m = 0; m < E; ++m)
             {
-                if (i < s)
+                part += v[m].x + v[m].y;
+            }
+            constexpr int W = T < 64 ? T : 64;
+#pragma unroll
+            for (int o = W / 2; o > 0; o >>= 1)
+            {
+                part += __shfl_xor(part, o, 64);
+            }
+            if constexpr (T > 64)
+            {
+                if ((tid & 63) == 0)
                 {
-                    red[tid] += red[tid + s];
+                    red[tid >> 6] = part;
+                }
+                __syncthreads();
+                const int w0 = (local * T) >> 6;
+                part = 0.0f;
+#pragma unroll
+                for (int w = 0; w < T / 64; ++w)
+                {
+                    part += red[w0 + w];
                 }
                 __syncthreads();
             }
-            const float mean = red[local * T] / static_cast<float>(2 * H);
-            __syncthreads();
+            const float mean = part / static_cast<float>(2 * H);
 #pragma unroll
             for (int m = 0; m < E; ++m)
             {
@@ -118,9 +147,9 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
 #pragma unroll
             for (int m = 0; m < E; ++m)
             {
-                const uint32_t j = static_cast<uint32_t>(i + T * m);
-                v[m].x *= bld32<0>(wr, 8u * j, 0);
-                v[m].y *= bld32<0>(wr, 8u * j + 4u, 0);
+                const cf wv = bld<0>(wr, 8u * static_cast<uint32_t>(i + T * m), 0);
+                v[m].x *= wv.x;
+                v[m].y *= wv.y;
             }
         }
         fft_core<H, E, false, 1>(v, i, view, tw);
@@ -143,8 +172,8 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
             const cf e = {(zk.x + zh.x) * 0.5f, (zk.y - zh.y) * 0.5f};
             const cf o = {(zk.y + zh.y) * 0.5f, (zh.x - zk.x) * 0.5f};
             const cf xk = cadd(e, cmul(o, wk[k]));
-            // k == 0 also yields the Nyquist bin: W_N^H = -1, X[H] = Re Z0 - Im Z0
-            const cf xn = cadd(e, cmul(o, wk[H]));
+            // k == 0 also yields the Nyquist bin: W_N^H = -1, X[H] = Re Z0 - Im Z0 (read only where k == 0)
+            const cf xn = {zk.x - zk.y, 0.0f};
             if constexpr (MODE == 0)
             {
                 if (active)
@@ -160,7 +189,7 @@ __global__ __launch_bounds__((SpecCfg<H, E>::threads)) void k_spec(SpecArgs a)
             else
             {
                 const float p = a.scale * (xk.x * xk.x + xk.y * xk.y) * (k == 0 ? 1.0f : 2.0f);
-                const float pn = a.scale * (xn.x * xn.x + xn.y * xn.y);
+                const float pn = a.scale * (xn.x * xn.x);
                 if constexpr (MODE == 1)
                 {
                     if (active)
