@@ -14,6 +14,7 @@
 
 #include "../../include/crdfft.h"
 #include "many.hpp"
+#include "scratch.hpp"
 
 namespace
 {
@@ -187,7 +188,7 @@ std::vector<crdfft_radix_plan> candidates(uint64_t n)
 struct DevBuf
 {
     void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 16)); }
+    hipError_t alloc(size_t bytes) { return crdfft::device_malloc(&p, std::max<size_t>(bytes, 16)); }
     ~DevBuf()
     {
         if (p != nullptr)

@@ -135,7 +135,7 @@ crdfft_status build(crdfft_nd_plan_s* p, const uint64_t* dims, uint32_t ndim, ui
         {
             a.kind = kColumns;
             const uint64_t entries = std::max<uint64_t>(1, crdfft::stage_twiddle_entries(a.d, a.e));
-            ND_TRY(hipMalloc(&a.tw, entries * sizeof(crdfft::cfloat)));
+            ND_TRY(crdfft::device_malloc(reinterpret_cast<void**>(&a.tw), entries * sizeof(crdfft::cfloat)));
             ND_TRY(crdfft::build_stage_twiddles(static_cast<crdfft::cfloat*>(a.tw), a.d, a.e, nullptr));
         }
         else

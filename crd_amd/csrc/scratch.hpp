@@ -5,13 +5,47 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <utility>
 #include <vector>
 
 namespace crdfft
 {
+
+// Fault seam for the allocation-failure tests (the reference's refusing allocator, fft.hpp:52): with
+// CRDFFT_FAULT_ALLOC=k in the environment, the k-th library-owned device allocation counted from the moment the
+// variable took that value fails as out of memory, so every error path of plan creation can be driven from the host
+// without exhausting the device. Unset, empty or 0: no effect.
+inline bool alloc_fault_forced()
+{
+    static std::mutex mutex;
+    static std::string last;
+    static uint64_t count = 0;
+    const char* v = std::getenv("CRDFFT_FAULT_ALLOC");
+    const std::string cur = v != nullptr ? v : "";
+    std::lock_guard<std::mutex> lock(mutex);
+    if (cur != last)
+    {
+        last = cur;
+        count = 0;
+    }
+    const long k = cur.empty() ? 0 : std::atol(cur.c_str());
+    return k > 0 && ++count == static_cast<uint64_t>(k);
+}
+
+// hipMalloc behind the fault seam: every device allocation a plan owns goes through here.
+inline hipError_t device_malloc(void** p, size_t bytes)
+{
+    if (alloc_fault_forced())
+    {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
 
 struct DeviceBuffer
 {
@@ -24,7 +58,7 @@ struct DeviceBuffer
         {
             return hipSuccess;
         }
-        return hipMalloc(&ptr, b);
+        return device_malloc(&ptr, b);
     }
     ~DeviceBuffer()
     {

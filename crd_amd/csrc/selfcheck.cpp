@@ -20,6 +20,7 @@
 #include "../../include/crdfft.h"
 #include "../../include/crdfft_runtime.h"
 #include "planner.hpp"
+#include "scratch.hpp"
 
 namespace
 {
@@ -100,7 +101,7 @@ extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
         const uint64_t cases = 3;
         std::vector<std::complex<float>> host(n), out(n);
         DevBuf din, dout;
-        if (hipMalloc(&din.p, n * 8) != hipSuccess || hipMalloc(&dout.p, n * 8) != hipSuccess)
+        if (crdfft::device_malloc(&din.p, n * 8) != hipSuccess || crdfft::device_malloc(&dout.p, n * 8) != hipSuccess)
         {
             return CRDFFT_OUT_OF_MEMORY;
         }
@@ -201,7 +202,7 @@ extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
     // R2C / C2R: a cosine of frequency 1 and its half spectrum
     const uint64_t h = n / 2;
     DevBuf dr, dc;
-    if (hipMalloc(&dr.p, n * 4) != hipSuccess || hipMalloc(&dc.p, (h + 1) * 8) != hipSuccess)
+    if (crdfft::device_malloc(&dr.p, n * 4) != hipSuccess || crdfft::device_malloc(&dc.p, (h + 1) * 8) != hipSuccess)
     {
         return CRDFFT_OUT_OF_MEMORY;
     }

@@ -333,6 +333,66 @@ def test_empty_batches_are_no_ops(gpu):
     assert not psd[:129].any() and np.array_equal(psd[129:], guard[129:])
 
 
+def test_allocation_failures_are_reported(gpu):
+    """CRDFFT_FAULT_ALLOC=k fails the k-th device allocation the library makes (the reference's refusing allocator,
+    fft.hpp:52, exercised by its fatal tests, tests/numerics/hesap-fft/CMakeLists.txt:26-36): each plan kind, failed
+    at every allocation in turn, returns OUT_OF_MEMORY and never crashes, or (a 2/3/5/7-smooth Bluestein plan, whose
+    mixed-radix launch falls back to chirp-z) still builds a plan that computes the right transform."""
+    from crd_amd import _native as N
+    from crd_amd import spectral
+    from crd_amd.dct import DctPlan
+
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((4, 2000), dtype=np.float32)).view(np.complex64)
+    makers = {
+        "c2c 4096": lambda: gpu.FftPlan(4096, 8),
+        "c2c 2^16": lambda: gpu.FftPlan(1 << 16, 8),
+        "c2c 2^20": lambda: gpu.FftPlan(1 << 20, 2),
+        "r2c 2^16": lambda: gpu.FftPlan(1 << 16, 4, "r2c"),
+        "c2r 2^18": lambda: gpu.FftPlan(1 << 18, 4, "c2r"),
+        "bluestein 1000": lambda: gpu.BluesteinPlan(1000, 4),
+        "bluestein 1009": lambda: gpu.BluesteinPlan(1009, 4),
+        "bluestein 10007 passes": lambda: gpu.BluesteinPlan(10007, 2),
+        "dct 4096": lambda: DctPlan(4096, 4),
+        "dct 2^16": lambda: DctPlan(1 << 16, 2),
+        "spectral": lambda: spectral.SpectralPlan(1024, 512),
+        "nd": lambda: gpu.NdFftPlan((16, 32, 64), 2),
+    }
+    try:
+        for name, make in makers.items():
+            failures = 0
+            for k in range(1, 25):
+                os.environ["CRDFFT_FAULT_ALLOC"] = str(k)
+                try:
+                    p = make()
+                except N.CrdfftError as e:
+                    assert e.status == N.OUT_OF_MEMORY, (name, k, e.status)
+                    failures += 1
+                    continue
+                finally:
+                    os.environ["CRDFFT_FAULT_ALLOC"] = "0"
+                if name == "bluestein 1000":
+                    buf = gpu.DeviceBuffer.from_array(x[:, :1000])
+                    p.execute(buf, buf, gpu.FORWARD)
+                    gpu.synchronize()
+                    y = buf.download((4, 1000), np.complex64)
+                    want = np.fft.fft(x[:, :1000].astype(np.complex128), axis=1)
+                    assert np.linalg.norm(y - want) / np.linalg.norm(want) <= 2e-6 * 10, (name, k)
+                del p
+            assert failures >= 1 or name == "bluestein 1000", name
+    finally:
+        del os.environ["CRDFFT_FAULT_ALLOC"]
+    # without the fault the same plans build and run
+    p = gpu.FftPlan(1 << 16, 2)
+    xin = (rng.standard_normal((2, 2 << 16), dtype=np.float32)).view(np.complex64)
+    buf = gpu.DeviceBuffer.from_array(xin)
+    p.execute(buf, buf, gpu.FORWARD)
+    gpu.synchronize()
+    y = buf.download(xin.shape, np.complex64)
+    want = np.fft.fft(xin.astype(np.complex128), axis=1)
+    assert np.linalg.norm(y - want) / np.linalg.norm(want) <= 2e-6 * 16
+
+
 def test_self_check_fault_seam(gpu):
     """CRDFFT_FAULT_SELF_CHECK=1 flips an output bit on the host: plan creation must fail with SELF_CHECK_FAILED
     (the reference forces its demotion path the same way, fault_seam.hpp:139)."""
