@@ -22,6 +22,7 @@ FLAG_NONE, FLAG_NO_SELF_CHECK, FLAG_NORMALIZE_INVERSE = 0, 1, 2
 FLAG_SEPARATE_PASSES = 4
 FLAG_CHIRP_Z = 8
 HINTS_NONE = 0x100
+HINT_TWO_PIPELINES = 0x20
 DCT2, DCT3, DST2, DST3 = 0, 1, 2, 3
 ND_TRANSPOSE_ALL = 1
 WINDOW_BOXCAR, WINDOW_HANN = 0, 1

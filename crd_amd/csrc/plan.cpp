@@ -60,7 +60,66 @@ crdfft_status hip_status(hipError_t e)
 
 using crdfft::cfloat;
 
-// A device buffer the plan owns.
+// The second pipeline of a two-pipeline four-step (kHintTwoPipes), one per caller stream: an internal stream at the
+// default priority, its own workspace, and the events that fork it from the caller's stream and join it back. It is
+// created on the first execute on that stream (which must not be inside a graph capture: it allocates); the fork and
+// join are ordinary event records and waits, so a later execute on the stream can be captured.
+struct AuxPipe
+{
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join = nullptr;
+    DeviceBuffer ws;
+    ~AuxPipe()
+    {
+        if (fork != nullptr)
+        {
+            (void)hipEventDestroy(fork);
+        }
+        if (join != nullptr)
+        {
+            (void)hipEventDestroy(join);
+        }
+        if (stream != nullptr)
+        {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+struct AuxPipes
+{
+    std::mutex mutex;
+    std::vector<std::pair<hipStream_t, std::unique_ptr<AuxPipe>>> all;
+    static constexpr size_t kMaxStreams = 64;
+
+    // The pipeline of caller stream s with a workspace of `bytes`; nullptr when it cannot be created.
+    AuxPipe* get(hipStream_t s, uint64_t bytes)
+    {
+        std::lock_guard<std::mutex> lock(mutex);
+        for (auto& e : all)
+        {
+            if (e.first == s)
+            {
+                return e.second.get();
+            }
+        }
+        if (all.size() >= kMaxStreams)
+        {
+            return nullptr;
+        }
+        auto a = std::make_unique<AuxPipe>();
+        if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&a->join, hipEventDisableTiming) != hipSuccess || a->ws.alloc(bytes) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        all.emplace_back(s, std::move(a));
+        return all.back().second.get();
+    }
+};
 } // namespace
 
 struct crdfft_plan_s
@@ -90,6 +149,7 @@ struct crdfft_plan_s
     uint64_t coop_ctl_offset = 0; // COOP: byte offset of the control words in the workspace
     int blue_e = 0; // > 0: the one-kernel form at this many values per thread (its stage table in tw_single)
     PerStream workspace_of, bwork_of; // per-stream copies of workspace / bwork
+    AuxPipes pipes;                   // per caller stream: the second pipeline of a two-pipeline four-step
     std::mutex ref_mutex;
     ~crdfft_plan_s()
     {
@@ -325,52 +385,83 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             {
                 return CRDFFT_OUT_OF_MEMORY;
             }
-            for (uint64_t c0 = 0; c0 < count; c0 += sh.chunk)
+            // two pipelines: odd chunks run on the caller stream's second pipeline with its own workspace, forked
+            // from and joined back to the caller's stream (profiled executes keep one stream: their events time
+            // each launch on it)
+            AuxPipe* aux = nullptr;
+            if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > sh.chunk && prof.acc == nullptr)
             {
-                const uint64_t cnt = std::min(sh.chunk, count - c0);
-                crdfft::ColArgs ca{in + c0 * idist,
-                                   ws,
-                                   p->tw1.as<cfloat>(),
-                                   p->thi.as<cfloat>(),
-                                   p->tlo.as<cfloat>(),
-                                   p->tstep.as<cfloat>(),
-                                   p->tfull.as<cfloat>(),
-                                   p->hshift,
-                                   sh.n2,
-                                   cnt,
-                                   idist,
-                                   cn,
-                                   sh.hints & crdfft::kHintNtLoad,
-                                   c2r_w,
-                                   1.0f / static_cast<float>(cn)};
-                CRDFFT_TRY(prof.before());
-                if (c2r_w != nullptr)
+                aux = p->pipes.get(s, p->workspace.bytes);
+                if (aux == nullptr)
                 {
-                    CRDFFT_TRY(crdfft::launch_cols_c2r(sh.n1, sh.e1, sh.tile1, ca, s));
+                    return CRDFFT_OUT_OF_MEMORY;
                 }
-                else if ((sh.hints & crdfft::kHintPersistent) != 0)
-                {
-                    CRDFFT_TRY(crdfft::launch_cols_pf(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
-                }
-                else
-                {
-                    CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, s));
-                }
-                CRDFFT_TRY(prof.after(0, cn * cnt));
-                crdfft::RowTrArgs ra{ws, out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
-                                     cn, odist, scale, sh.hints & crdfft::kHintNtStore, r2c_w};
-                CRDFFT_TRY(prof.before());
-                if (r2c_w != nullptr)
-                {
-                    CRDFFT_TRY(crdfft::launch_rowtr_r2c(sh.n2, sh.e2, sh.tile2, ra, s));
-                }
-                else
-                {
-                    CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, s));
-                }
-                CRDFFT_TRY(prof.after(1, cn * cnt));
+                CRDFFT_TRY(hipEventRecord(aux->fork, s));
+                CRDFFT_TRY(hipStreamWaitEvent(aux->stream, aux->fork, 0));
             }
-            return CRDFFT_OK;
+            const hipStream_t caller = s;
+            const crdfft_status st = [&]() -> crdfft_status {
+                for (uint64_t c0 = 0; c0 < count; c0 += sh.chunk)
+                {
+                    const uint64_t cnt = std::min(sh.chunk, count - c0);
+                    const bool second = aux != nullptr && ((c0 / sh.chunk) & 1) != 0;
+                    const hipStream_t cs = second ? aux->stream : caller;
+                    cfloat* const ws_c = second ? aux->ws.as<cfloat>() : ws;
+                    crdfft::ColArgs ca{in + c0 * idist,
+                                       ws_c,
+                                       p->tw1.as<cfloat>(),
+                                       p->thi.as<cfloat>(),
+                                       p->tlo.as<cfloat>(),
+                                       p->tstep.as<cfloat>(),
+                                       p->tfull.as<cfloat>(),
+                                       p->hshift,
+                                       sh.n2,
+                                       cnt,
+                                       idist,
+                                       cn,
+                                       sh.hints & crdfft::kHintNtLoad,
+                                       c2r_w,
+                                       1.0f / static_cast<float>(cn)};
+                    CRDFFT_TRY(prof.before());
+                    if (c2r_w != nullptr)
+                    {
+                        CRDFFT_TRY(crdfft::launch_cols_c2r(sh.n1, sh.e1, sh.tile1, ca, cs));
+                    }
+                    else if ((sh.hints & crdfft::kHintPersistent) != 0)
+                    {
+                        CRDFFT_TRY(crdfft::launch_cols_pf(sh.n1, sh.e1, sh.tile1, inverse, ca, cs));
+                    }
+                    else
+                    {
+                        CRDFFT_TRY(crdfft::launch_cols(sh.n1, sh.e1, sh.tile1, inverse, ca, cs));
+                    }
+                    CRDFFT_TRY(prof.after(0, cn * cnt));
+                    crdfft::RowTrArgs ra{ws_c, out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
+                                         cn, odist, scale, sh.hints & crdfft::kHintNtStore, r2c_w};
+                    CRDFFT_TRY(prof.before());
+                    if (r2c_w != nullptr)
+                    {
+                        CRDFFT_TRY(crdfft::launch_rowtr_r2c(sh.n2, sh.e2, sh.tile2, ra, cs));
+                    }
+                    else
+                    {
+                        CRDFFT_TRY(crdfft::launch_rowtr(sh.n2, sh.e2, sh.tile2, inverse, ra, cs));
+                    }
+                    CRDFFT_TRY(prof.after(1, cn * cnt));
+                }
+                return CRDFFT_OK;
+            }();
+            if (aux != nullptr)
+            {
+                // joined back even after a failed launch, so nothing on the second stream outlives the caller's order
+                const hipError_t e1 = hipEventRecord(aux->join, aux->stream);
+                const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(caller, aux->join, 0) : e1;
+                if (st == CRDFFT_OK && e2 != hipSuccess)
+                {
+                    return hip_status(e2);
+                }
+            }
+            return st;
         }
         case CRDFFT_ALGO_COOP:
         {

@@ -41,9 +41,15 @@ constexpr uint64_t kFourStepMax = uint64_t{1} << 24;
 // and those slower than 256 MiB at n = 2^16; the intermediate does not measurably stay in the Infinity Cache).
 constexpr uint64_t kChunkBytes = uint64_t{256} << 20;
 // Plan hints of the four-step: bit 0 streams the column pass's input past the caches, bit 1 the row pass's output;
-// bit 3 runs the column pass as the persistent prefetching launch.
+// bit 3 runs the column pass as the persistent prefetching launch; bit 5 runs alternate chunks on a second stream
+// (two column/row pipelines in flight, each with its own half-size workspace).
 constexpr uint32_t kDefaultFourStepHints = 3;
 constexpr uint32_t kHintPersistentColumns = 8;
+constexpr uint32_t kHintTwoPipes = 32;
+// Columns this long make the column pass latency-bound (one workgroup fills a CU's LDS): such C2C plans default to
+// two pipelines, so one chunk's row pass runs beside the next chunk's column pass. Measured on MI355X, back-to-back
+// executes: n = 2^20 x 64 5 % less time, 2^22 x 16 4 %, 2^24 x 4 2.5 %; 2^18 x 256 (1024-point columns) 2 % more.
+constexpr uint64_t kTwoPipesMinN1 = 2048;
 // The largest complex length of the fused four-step (one transform's intermediate per workgroup slot).
 constexpr uint64_t kFusedMax = uint64_t{1} << 17;
 // Workgroup size of the fused four-step unless tuned: 512 threads (two workgroups per CU hide each other's memory

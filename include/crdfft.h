@@ -118,11 +118,18 @@ typedef struct crdfft_plan_info
     uint64_t chunk;       /* transforms per column+row pass pair (four-step) */
     uint64_t workspace_bytes;
     uint64_t table_bytes;
-    uint32_t hints;       /* cache hints: bit 0 non-temporal input loads, bit 1 non-temporal output stores */
+    uint32_t hints;       /* cache hints: bit 0 non-temporal input loads, bit 1 non-temporal output stores;
+                             four-step bit 5 (CRDFFT_HINT_TWO_PIPELINES): alternate chunks run on a second stream */
 } crdfft_plan_info;
 
 /* Tuning overrides for crdfft_plan_create_tuned; a zero field keeps the planner's choice. */
 #define CRDFFT_HINTS_NONE 0x100u /* crdfft_tuning.hints: no cache hints (plain loads and stores) */
+/* Four-step plans: alternate chunks run on a second stream of the plan's (one per caller stream, forked from and
+ * joined back to the caller's stream with events), each chunk pipeline with its own workspace, so one chunk's row
+ * pass overlaps the next chunk's column pass. The planner sets it for C2C plans with columns of 2048 points and more
+ * and at least two chunks; the first execute on a caller stream creates that stream's pipeline (not inside a graph
+ * capture); later executes on it can be captured. */
+#define CRDFFT_HINT_TWO_PIPELINES 0x20u
 typedef struct crdfft_tuning
 {
     uint64_t n1;          /* four-step split */

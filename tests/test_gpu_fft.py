@@ -333,6 +333,36 @@ def test_empty_batches_are_no_ops(gpu):
     assert not psd[:129].any() and np.array_equal(psd[129:], guard[129:])
 
 
+@pytest.mark.parametrize("lg,batch,chunk", [(18, 40, 7), (20, 8, 2), (20, 5, 2), (22, 3, 1)])
+def test_two_pipelines_match_one(gpu, lg, batch, chunk):
+    """Long-column four-step plans run alternate chunks on a second stream, forked from and joined back to the
+    caller's: the output is bit-identical to the same kernels on one stream (odd chunk counts, a partial last chunk,
+    both directions, a second caller stream and the null stream), and matches FP64."""
+    n = 1 << lg
+    two = gpu.FftPlan(n, batch, tuning={"chunk": chunk, "hints": 3 | 32})
+    one = gpu.FftPlan(n, batch, tuning={"chunk": chunk, "hints": 3})
+    assert two.info()["hints"] & 32 and not one.info()["hints"] & 32 and two.info()["chunk"] == chunk
+    x = (np.random.default_rng(lg).standard_normal((batch, 2 * n), dtype=np.float32)).view(np.complex64)
+    src = gpu.DeviceBuffer.from_array(x)
+    a, b = gpu.DeviceBuffer(x.nbytes), gpu.DeviceBuffer(x.nbytes)
+    other = gpu.Stream()
+    for direction in (gpu.FORWARD, gpu.INVERSE):
+        one.execute(src, a, direction)
+        for stream in (None, other):
+            b.fill(0x7F, stream)
+            two.execute(src, b, direction, stream)
+            if stream is not None:
+                stream.synchronize()
+            gpu.synchronize()
+            assert np.array_equal(a.download(x.shape, np.uint64), b.download(x.shape, np.uint64)), (direction, stream)
+    gpu.FftPlan(n, batch).execute(src, b, gpu.FORWARD)  # the planner's own choice
+    gpu.synchronize()
+    y = b.download(x.shape, np.complex64)
+    rows = min(batch, 2)
+    want = np.fft.fft(x[:rows].astype(np.complex128), axis=1)
+    assert np.linalg.norm(y[:rows] - want) / np.linalg.norm(want) <= 2e-6 * lg
+
+
 def test_allocation_failures_are_reported(gpu):
     """CRDFFT_FAULT_ALLOC=k fails the k-th device allocation the library makes (the reference's refusing allocator,
     fft.hpp:52, exercised by its fatal tests, tests/numerics/hesap-fft/CMakeLists.txt:26-36): each plan kind, failed

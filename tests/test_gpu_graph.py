@@ -1,5 +1,6 @@
 """hipGraph capture of the plans' executes (crdfft_graph_*): one graph holding a C2C single-kernel and a four-step
-transform, the cooperative single pass, R2C/C2R in one kernel and in two fused passes, a DCT, an any-length C2C
+transform, the cooperative single pass, a two-pipeline four-step (a second stream forked and joined inside the
+capture), R2C/C2R in one kernel and in two fused passes, a DCT, an any-length C2C
 (mixed radix and chirp-z) and an STFT, replayed several times with eager work and synchronisations between the
 replays; every replay must reproduce the eager results bit for bit (the same kernels on the same inputs).
 """
@@ -31,6 +32,7 @@ def test_graph_replays_match_eager(gpu, po):
     c2c("c2c 4096", 4096, 64)
     c2c("c2c 2^16 four-step", 1 << 16, 8)
     c2c("c2c 2^16 cooperative", 1 << 16, 4, {"algorithm": 6})
+    c2c("c2c 2^20 two pipelines", 1 << 20, 5, {"chunk": 2, "hints": 3 | 32})  # fork/join of a second stream
     for n in (4096, 1 << 16):
         x = rng.standard_normal((4, n), dtype=np.float32)
         r2c = gpu.FftPlan(n, 4, "r2c")
