@@ -81,6 +81,41 @@ def test_planner_defaults_follow_the_measured_tables():
         assert crd_amd.describe(1 << lg, 4)["e2"] == e2, lg
 
 
+def test_planner_four_step_schedules():
+    """The measured four-step schedules: 256 MiB chunks in one pipeline up to 1024-point columns; two pipelines of
+    128 MiB chunks from 2048-point columns when the batch needs at least two of them; C2R at half-size chunks; R2C
+    without cache hints; the n = 2^16 real splits."""
+    import crd_amd
+    from crd_amd import _native as N
+
+    two = N.HINT_TWO_PIPELINES
+    d = crd_amd.describe(1 << 16, 4096)
+    assert (d["n1"], d["chunk"], d["hints"]) == (256, 512, 3)
+    d = crd_amd.describe(1 << 18, 1024)
+    assert (d["n1"], d["chunk"], d["hints"]) == (1024, 128, 3)
+    d = crd_amd.describe(1 << 20, 64)
+    assert (d["n1"], d["chunk"], d["hints"]) == (2048, 16, 3 | two)
+    d = crd_amd.describe(1 << 20, 16)  # one chunk: one pipeline
+    assert (d["chunk"], d["hints"]) == (16, 3)
+    d = crd_amd.describe(1 << 24, 4)
+    assert (d["n1"], d["chunk"], d["hints"]) == (4096, 1, 3 | two)
+    assert crd_amd.describe(1 << 20, 64, "r2c")["hints"] & two == 0
+    d = crd_amd.describe(1 << 16, 4096, "c2r")
+    assert (d["n1"], d["n2"], d["e2"], d["chunk"], d["hints"]) == (256, 128, 8, 512, 3)
+    d = crd_amd.describe(1 << 20, 64, "c2r")
+    assert (d["chunk"], d["hints"]) == (32, 3)
+    d = crd_amd.describe(1 << 16, 4096, "r2c")
+    assert (d["n1"], d["n2"], d["e2"], d["tile2"], d["chunk"], d["hints"]) == (128, 256, 8, 32, 1024, 0)
+    # a tuned plan can switch the pipelines either way
+    info = N.PlanInfo()
+    t = N.Tuning(hints=3)
+    assert N.lib().crdfft_plan_describe_tuned(1 << 20, 64, 0, 0, ctypes.byref(t), ctypes.byref(info)) == N.OK
+    assert info.hints == 3 and info.chunk == 32
+    t = N.Tuning(hints=3 | two, chunk=4)
+    assert N.lib().crdfft_plan_describe_tuned(1 << 18, 64, 0, 0, ctypes.byref(t), ctypes.byref(info)) == N.OK
+    assert info.hints == 3 | two and info.chunk == 4
+
+
 def test_tuning_hints_none_is_explicit():
     """CRDFFT_HINTS_NONE asks for plain loads and stores; 0 keeps the planner's default."""
     import crd_amd
