@@ -158,7 +158,7 @@ def bench_mixed(stream, Event):
     for name, ch in (("tuned", choices), ("default_radix_plans", None)):
         plan = many.ManyPlan(lengths, choices=ch)
         plan.capture_graph(src, dst, crd_amd.FORWARD, stream)
-        ms = time_loop(lambda: plan.launch_graph(stream), stream, 20, 3, Event)
+        ms = time_loop(lambda: plan.launch_graph(stream), stream, 20, 10, Event)
         out[name] = {"ms": round(ms, 4), "ffts_per_s": round(len(lengths) / (ms * 1e-3), 1),
                      "gbs": round(16.0 * total / (ms * 1e-3) / 1e9, 1)}
         if name == "tuned":
@@ -224,18 +224,18 @@ def bench_spectral(stream, Event):
     rows = DeviceBuffer(frames * nfreq * 8)
     read_bytes = 4.0 * frames * nperseg  # each frame's samples once (the overlap reads the signal twice)
     out = {"signal_samples": length, "nperseg": nperseg, "noverlap": noverlap, "frames": frames}
-    ms = time_loop(lambda: wplan.welch(dx, length, 1.0, psd, stream), stream, 10, 2, Event)
+    ms = time_loop(lambda: wplan.welch(dx, length, 1.0, psd, stream), stream, 10, 10, Event)
     out["welch"] = {"ms": round(ms, 4), "frames_per_s": round(frames / (ms * 1e-3), 1),
                     "gbs_frame_reads": round(read_bytes / (ms * 1e-3) / 1e9, 1)}
-    ms = time_loop(lambda: wplan.spectrogram(dx, length, 1.0, rows, stream), stream, 10, 2, Event)
+    ms = time_loop(lambda: wplan.spectrogram(dx, length, 1.0, rows, stream), stream, 10, 10, Event)
     out["spectrogram"] = {"ms": round(ms, 4),
                           "gbs": round((read_bytes + 4.0 * frames * nfreq) / (ms * 1e-3) / 1e9, 1)}
-    ms = time_loop(lambda: splan.stft(dx, length, rows, stream), stream, 10, 2, Event)
+    ms = time_loop(lambda: splan.stft(dx, length, rows, stream), stream, 10, 10, Event)
     out["stft"] = {"ms": round(ms, 4), "gbs": round((read_bytes + 8.0 * frames * nfreq) / (ms * 1e-3) / 1e9, 1)}
     # the FFT alone on a framed, windowed copy (what a frame-by-frame port would run after its own framing pass)
     framed = DeviceBuffer(frames * nperseg * 4)
     r2c = FftPlan(nperseg, frames, "r2c", self_check=False)
-    ms = time_loop(lambda: r2c.execute(framed, rows, crd_amd.FORWARD, stream), stream, 10, 2, Event)
+    ms = time_loop(lambda: r2c.execute(framed, rows, crd_amd.FORWARD, stream), stream, 10, 10, Event)
     out["r2c_of_framed_copy_ms"] = round(ms, 4)
     out["welch_vs_r2c_alone"] = round(out["r2c_of_framed_copy_ms"] / out["welch"]["ms"], 2)
     return out
@@ -259,7 +259,7 @@ def bench_r2r(stream, Event):
         plan = DctPlan(n, rows, passes=passes)
         form = {}
         for kind in range(4):
-            ms = time_loop(lambda: plan.execute(kind, src, dst, stream), stream, 20, 3, Event)
+            ms = time_loop(lambda: plan.execute(kind, src, dst, stream), stream, 20, 10, Event)
             form[names[kind]] = {"ms": round(ms, 4), "gbs": round(8.0 * n * rows / (ms * 1e-3) / 1e9, 1)}
         out["separate_passes" if passes else "one_kernel"] = form
     out["one_kernel_vs_passes_dct2"] = round(out["separate_passes"]["dct2"]["ms"] / out["one_kernel"]["dct2"]["ms"], 2)
@@ -269,7 +269,7 @@ def bench_r2r(stream, Event):
     blue = {"n": bn, "batch": bb}
     for name, kw in (("mixed_radix", {}), ("one_kernel", {"chirp_z": True}), ("separate_passes", {"passes": True})):
         plan = crd_amd.BluesteinPlan(bn, bb, **kw)
-        ms = time_loop(lambda: plan.execute(bsrc, bdst, crd_amd.FORWARD, stream), stream, 20, 3, Event)
+        ms = time_loop(lambda: plan.execute(bsrc, bdst, crd_amd.FORWARD, stream), stream, 20, 10, Event)
         blue[name] = {"ms": round(ms, 4), "gbs": round(16.0 * bn * bb / (ms * 1e-3) / 1e9, 1)}
     blue["one_kernel_vs_passes"] = round(blue["separate_passes"]["ms"] / blue["one_kernel"]["ms"], 2)
     out["bluestein"] = blue
@@ -298,7 +298,7 @@ def bench_split(stream, Event):
     out = {"n": n, "batch": batch}
     for prec in ("fp32", "split_bf16", "split_fp16"):
         p = crd_amd.FftPlan(n, batch, precision=prec)
-        ms = time_loop(lambda: p.execute(src, dst, crd_amd.FORWARD, stream), stream, 5, 1, Event)
+        ms = time_loop(lambda: p.execute(src, dst, crd_amd.FORWARD, stream), stream, 5, 5, Event)
         y = dst.download((4, n), np.complex64)
         out[prec] = {"ms": round(ms, 4), "ffts_per_s": round(batch / (ms * 1e-3), 1),
                      "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
@@ -486,7 +486,7 @@ def main():
                 r2c.execute(real_in, spec, crd_amd.FORWARD, stream)
                 c2r.execute(spec, back, crd_amd.INVERSE, stream)
 
-            ms = time_loop(rt, stream, max(3, args.steps // 2), 1, Event)
+            ms = time_loop(rt, stream, max(3, args.steps // 2), args.warmup, Event)
             bytes_rt = float(batch) * (4 * n + 8 * (n // 2 + 1)) * 2
             m = min(4, batch)
             xb = back.download((m, n), np.float32)
@@ -509,7 +509,7 @@ def main():
             state["i"] += 1
             p.execute(ins[i], outs[i], crd_amd.FORWARD, stream)
 
-        ms = time_loop(rot, stream, 40, 4, Event)
+        ms = time_loop(rot, stream, 40, 10, Event)
         extras["c2c_n4096_b1024"] = {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
                                      "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
         extras["mixed_size"] = bench_mixed(stream, Event)
