@@ -365,9 +365,10 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
             s.hints = tuning_hints(tuning->hints, 11u | kHintTwoPipes);
         }
         // two pipelines keep two chunks in flight: each gets half the workspace budget. C2R, whose column pass reads
-        // every bin twice (X[j] and X[h - j]), also runs half-size chunks: measured 3-11 % faster at n = 2^16..2^22
-        const uint64_t budget =
-            ((s.hints & kHintTwoPipes) != 0 || type == CRDFFT_C2R) ? kChunkBytes / 2 : kChunkBytes;
+        // every bin twice (X[j] and X[h - j]), also runs half-size chunks: measured 3-11 % faster at n = 2^16..2^22;
+        // so does R2C from 1024-point columns (n = 2^20: 5 % faster; 2^22: equal)
+        const bool half = (s.hints & kHintTwoPipes) != 0 || type == CRDFFT_C2R || (type == CRDFFT_R2C && s.n1 >= 1024);
+        const uint64_t budget = half ? kChunkBytes / 2 : kChunkBytes;
         s.chunk = std::max<uint64_t>(1, budget / per);
         if (tuning != nullptr && tuning->chunk != 0)
         {
