@@ -117,9 +117,12 @@ std::vector<crdfft_tuning> enumerate(uint64_t n, uint64_t batch, int32_t type)
             {
                 continue;
             }
+            // from 2048-point columns also the two-pipeline schedule
+            const std::vector<uint32_t> hint_set =
+                l1 >= 11 ? std::vector<uint32_t>{1u, 3u, 3u | crdfft::kHintTwoPipes} : std::vector<uint32_t>{1u, 3u};
             for (uint64_t mib : {64u, 256u, 1024u})
             {
-                for (uint32_t hints : {1u, 3u})
+                for (uint32_t hints : hint_set)
                 {
                     for (int variant = 0; variant < 6; ++variant)
                     {
@@ -364,7 +367,8 @@ crdfft_status crdfft_plan_features(const crdfft_plan_record* records, uint64_t c
         f[20] = log2f_safe(static_cast<double>(r.tuning.chunk) + 1.0) / 12.0f;
         f[21] = (r.tuning.hints & 1u) ? 1.0f : 0.0f;
         f[22] = (r.tuning.hints & 2u) ? 1.0f : 0.0f;
-        f[23] = (r.tuning.hints & 4u) ? 1.0f : 0.0f; // N-D: every axis through transposed lines
+        // the schedule bit: N-D every axis through transposed lines, four-step two chunk pipelines
+        f[23] = (r.tuning.hints & (4u | crdfft::kHintTwoPipes)) ? 1.0f : 0.0f;
         f[24] = static_cast<float>(r.tuning.e1) / 32.0f;
         f[25] = static_cast<float>(r.tuning.e2) / 32.0f;
         f[26] = static_cast<float>(r.ndim) / 3.0f;
