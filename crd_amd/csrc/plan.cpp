@@ -93,8 +93,9 @@ struct AuxPipes
     std::vector<std::pair<hipStream_t, std::unique_ptr<AuxPipe>>> all;
     static constexpr size_t kMaxStreams = 64;
 
-    // The pipeline of caller stream s with a workspace of `bytes`; nullptr when it cannot be created.
-    AuxPipe* get(hipStream_t s, uint64_t bytes)
+    // The pipeline of caller stream s with a workspace of `bytes`; nullptr when it cannot be created, or when it does
+    // not exist yet and `create` is false.
+    AuxPipe* get(hipStream_t s, uint64_t bytes, bool create)
     {
         std::lock_guard<std::mutex> lock(mutex);
         for (auto& e : all)
@@ -104,7 +105,7 @@ struct AuxPipes
                 return e.second.get();
             }
         }
-        if (all.size() >= kMaxStreams)
+        if (!create || all.size() >= kMaxStreams)
         {
             return nullptr;
         }
@@ -391,13 +392,20 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             AuxPipe* aux = nullptr;
             if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > sh.chunk && prof.acc == nullptr)
             {
-                aux = p->pipes.get(s, p->workspace.bytes);
-                if (aux == nullptr)
+                // a stream being captured cannot allocate its pipeline: without one it runs both halves in order
+                hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
+                CRDFFT_TRY(hipStreamIsCapturing(s, &capturing));
+                const bool create = capturing == hipStreamCaptureStatusNone;
+                aux = p->pipes.get(s, p->workspace.bytes, create);
+                if (aux == nullptr && create)
                 {
                     return CRDFFT_OUT_OF_MEMORY;
                 }
-                CRDFFT_TRY(hipEventRecord(aux->fork, s));
-                CRDFFT_TRY(hipStreamWaitEvent(aux->stream, aux->fork, 0));
+                if (aux != nullptr)
+                {
+                    CRDFFT_TRY(hipEventRecord(aux->fork, s));
+                    CRDFFT_TRY(hipStreamWaitEvent(aux->stream, aux->fork, 0));
+                }
             }
             const hipStream_t caller = s;
             const crdfft_status st = [&]() -> crdfft_status {

@@ -127,8 +127,8 @@ typedef struct crdfft_plan_info
 /* Four-step plans: alternate chunks run on a second stream of the plan's (one per caller stream, forked from and
  * joined back to the caller's stream with events), each chunk pipeline with its own workspace, so one chunk's row
  * pass overlaps the next chunk's column pass. The planner sets it for C2C plans with columns of 2048 points and more
- * and at least two chunks; the first execute on a caller stream creates that stream's pipeline (not inside a graph
- * capture); later executes on it can be captured. */
+ * and at least two chunks; the first execute on a caller stream creates that stream's pipeline, and an execute
+ * captured into a graph before the pipeline exists records both halves on the captured stream instead. */
 #define CRDFFT_HINT_TWO_PIPELINES 0x20u
 typedef struct crdfft_tuning
 {

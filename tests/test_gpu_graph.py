@@ -88,3 +88,34 @@ def test_graph_replays_match_eager(gpu, po):
     finally:
         assert lib.crdfft_graph_destroy(graph) == N.OK
     assert lib.crdfft_graph_begin(None) == N.INVALID_ARGUMENT
+
+
+def test_two_pipeline_plan_first_run_inside_a_capture(gpu):
+    """A two-pipeline plan executed for the first time inside a capture cannot create the captured stream's second
+    pipeline (that allocates): it records both halves on the captured stream, and its replays match an eager run of
+    the same plan shape with the two pipelines, bit for bit."""
+    from crd_amd import _native as N
+
+    lib = N.lib()
+    n, batch = 1 << 20, 5
+    tuning = {"chunk": 2, "hints": 3 | 32}
+    eager_plan = gpu.FftPlan(n, batch, tuning=tuning)
+    captured_plan = gpu.FftPlan(n, batch, tuning=tuning)
+    x = (np.random.default_rng(9).standard_normal((batch, 2 * n), dtype=np.float32)).view(np.complex64)
+    src = gpu.DeviceBuffer.from_array(x)
+    want, got = gpu.DeviceBuffer(x.nbytes), gpu.DeviceBuffer(x.nbytes)
+    s = gpu.Stream()
+    eager_plan.execute(src, want, gpu.FORWARD, s)
+    s.synchronize()
+    assert lib.crdfft_graph_begin(s.handle) == N.OK
+    captured_plan.execute(src, got, gpu.FORWARD, s)  # its first execute: the plan's own workspace, one pipeline
+    graph = ctypes.c_void_p()
+    assert lib.crdfft_graph_end(s.handle, ctypes.byref(graph)) == N.OK
+    try:
+        for _ in range(2):
+            assert lib.crdfft_fill_bytes(got.address, 0x5A, x.nbytes, s.handle) == N.OK
+            assert lib.crdfft_graph_launch(graph, s.handle) == N.OK
+            s.synchronize()
+            assert np.array_equal(got.download(x.shape, np.uint64), want.download(x.shape, np.uint64))
+    finally:
+        assert lib.crdfft_graph_destroy(graph) == N.OK
