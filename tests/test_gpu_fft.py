@@ -355,6 +355,12 @@ def test_two_pipelines_match_one(gpu, lg, batch, chunk):
                 stream.synchronize()
             gpu.synchronize()
             assert np.array_equal(a.download(x.shape, np.uint64), b.download(x.shape, np.uint64)), (direction, stream)
+    # in place: chunk c + 1's column pass (reading its own transforms) runs beside chunk c's row pass (writing its own)
+    one.execute(src, a, gpu.FORWARD)
+    b.copy_from(src)
+    two.execute(b, b, gpu.FORWARD)
+    gpu.synchronize()
+    assert np.array_equal(a.download(x.shape, np.uint64), b.download(x.shape, np.uint64)), "in place"
     gpu.FftPlan(n, batch).execute(src, b, gpu.FORWARD)  # the planner's own choice
     gpu.synchronize()
     y = b.download(x.shape, np.complex64)
