@@ -62,8 +62,9 @@ using crdfft::cfloat;
 
 // The second pipeline of a two-pipeline four-step (kHintTwoPipes), one per caller stream: an internal stream at the
 // default priority, its own workspace, and the events that fork it from the caller's stream and join it back. It is
-// created on the first execute on that stream (which must not be inside a graph capture: it allocates); the fork and
-// join are ordinary event records and waits, so a later execute on the stream can be captured.
+// created on the first execute on that stream outside a graph capture (it allocates; a captured execute without one
+// runs both halves on the captured stream); the fork and join are ordinary event records and waits, so later
+// executes on the stream can be captured with it.
 struct AuxPipe
 {
     hipStream_t stream = nullptr;
