@@ -468,6 +468,25 @@ def smooth7(n):
     return n == 1
 
 
+def test_bluestein_large_prime_over_two_pipelines(gpu):
+    """A prime n = 1000003: the chirp-z passes over a 2^21-point plan whose long columns run two chunk pipelines
+    (16 transforms per pass chunk, 8 per pipeline chunk; a last pass chunk of 4 runs on one), against FP64."""
+    n, batch = 1000003, 20
+    plan = gpu.BluesteinPlan(n, batch)
+    assert plan.conv_size() == 1 << 21
+    x = (np.random.default_rng(11).standard_normal((batch, 2 * n), dtype=np.float32)).view(np.complex64)
+    src = gpu.DeviceBuffer.from_array(x)
+    dst = gpu.DeviceBuffer(x.nbytes)
+    for inverse in (False, True):
+        plan.execute(src, dst, gpu.INVERSE if inverse else gpu.FORWARD)
+        gpu.synchronize()
+        y = dst.download(x.shape, np.complex64)
+        for r in (0, 17, 19):
+            xr = x[r].astype(np.complex128)
+            ref = np.fft.ifft(xr) if inverse else np.fft.fft(xr)
+            assert np.linalg.norm(y[r] - ref) / np.linalg.norm(ref) <= 2 * bound(n), (r, inverse)
+
+
 @pytest.mark.parametrize("mode", ["default", "chirp_z", "passes"])
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 97, 100, 1000, 4097, 8191, 8192, 10007, 7 * 11 * 13 * 17,
                                3 * 5 * 7 * 11 * 13])
