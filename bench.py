@@ -27,6 +27,7 @@ sys.path.insert(0, HERE)
 METRIC = "batched C2C GB/s (% HBM peak); tuned vs plain FP32 FFTs/s at rel-L2≤1e-6"
 HBM_PEAK_GBS = 8000.0
 SHAPES = [(1 << 16, 4096), (1 << 20, 64)]
+EXTRAS = ("copy", "roundtrip", "c2c4096", "mixed", "split", "spectral", "r2r")
 
 
 def log(msg):
@@ -87,6 +88,14 @@ def rel_l2(y, ref):
     return float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
 
 
+def step_bytes_per_core_estimate(cores):
+    """GB/s of the headline step mix at the reference's published AVX-512 single-thread times, scaled by cores."""
+    published_ms = {1 << 16: 0.08764, 1 << 20: 2.66472}
+    total_bytes = sum(16.0 * n * b for n, b in SHAPES)
+    total_s = sum(published_ms[n] * 1e-3 * b for n, b in SHAPES) / cores
+    return total_bytes / total_s / 1e9
+
+
 def cpu_baseline(seconds_per_shape, threads):
     """The oracle's FP32 restatement on a bounded sample of the same shapes, on `threads` host threads."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -135,7 +144,40 @@ def cpu_baseline(seconds_per_shape, threads):
         "sample": "FP32 Stockham/four-step restatement (oracle/), one plan per thread over contiguous ranges: "
                   + "; ".join(parts) + "; scaled to one step of the GPU workload",
         "single_core_ms_per_n4096": round(1000.0 * best1 / 256, 5),
+        # the reference's own generated AVX-512 codelets run this transform in 0.00292 ms on one core
+        # (level_plans_gen.hpp:83); the portable restatement is this many times slower per core
+        "single_core_vs_reference_published": round(1000.0 * best1 / 256 / 0.00292, 2),
+        "reference_published_estimate": {
+            "value": round(step_bytes_per_core_estimate(threads), 2), "unit": "GB/s",
+            "what": "the reference's published single-core rates (0.08764 ms per 2^16 and 2.66472 ms per 2^20 "
+                    "transform, level_plans_gen.hpp:87, :91) times this host's cores, for the same step mix; an "
+                    "estimate, not a measurement"},
     }
+
+
+def bench_c2c4096(stream, Event):
+    """n = 4096 x 1024 (the reference's config-1 shape, 64 MiB) over 4 rotating buffer pairs so the set cannot sit
+    in the 256 MB Infinity Cache; 16 B per point algorithmic; one single-pass kernel per execute."""
+    import crd_amd
+    from crd_amd import DeviceBuffer, FftPlan
+
+    n, batch, nb = 4096, 1024, 4
+    p = FftPlan(n, batch)
+    ins = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
+    outs = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
+    for i, b in enumerate(ins):
+        fill_normal(b, 2 * n * batch, seed=3000 + i)
+    state = {"i": 0}
+
+    def rot():
+        i = state["i"] % nb
+        state["i"] += 1
+        p.execute(ins[i], outs[i], crd_amd.FORWARD, stream)
+
+    ms = time_loop(rot, stream, 40, 10, Event)
+    return {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
+            "frac_of_peak": round(16.0 * n * batch / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
 
 
 def bench_mixed(stream, Event):
@@ -321,6 +363,8 @@ def main():
                     help="normal: seeded N(0,1) uploaded from the host; device: a byte fill on the device (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB streaming copy (the HBM-counter calibration kernel)")
+    ap.add_argument("--extras", default=",".join(EXTRAS),
+                    help="comma list of the extra configurations to run (" + ", ".join(EXTRAS) + ")")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -399,26 +443,43 @@ def main():
         info = p.info()
         first_phase = {1: "single", 5: "fused four-step (both passes)"}.get(info["algorithm"], "column pass")
         names = [first_phase, "row pass (transposed store)", "real", "copy"]
+        # algorithmic bytes per complex point of one launch (one read + one write of the transform's input and
+        # output, intermediates not counted): 16 for a single-pass kernel, 8 for each pass of the four-step (the
+        # column pass reads the input, the row pass writes the output; the intermediate between them is excluded)
+        two_pass = info["algorithm"] == 2
         kernels = []
         for ph in range(4):
             if acc.launches[ph] == 0:
                 continue
             avg_ms = acc.ms[ph] / acc.launches[ph]
             pts = acc.points[ph] / acc.launches[ph]
-            ach = 16.0 * pts / (avg_ms * 1e-3) / 1e9
+            bpp = 8.0 if (two_pass and ph in (0, 1)) else 16.0
+            ach = bpp * pts / (avg_ms * 1e-3) / 1e9
             k = {"phase": names[ph], "launches_per_call": acc.launches[ph] // 3, "avg_ms": round(avg_ms, 5),
-                 "points_per_launch": int(pts), "achieved_gbs": round(ach, 1), "total_ms": acc.ms[ph] / 3}
+                 "points_per_launch": int(pts), "algorithmic_bytes_per_point": bpp,
+                 "achieved_gbs": round(ach, 1), "l2_miss_bytes_per_point_moved": 16.0,
+                 "total_ms": acc.ms[ph] / 3}
             kernels.append(k)
             if best is None or k["total_ms"] > best[0]["total_ms"]:
                 best = (k, n, batch, info)
+        if two_pass and len(kernels) >= 2:
+            # the transform as a whole: 16 B per point over the sum of its two passes' launch times
+            pts = kernels[0]["points_per_launch"]
+            both = kernels[0]["avg_ms"] + kernels[1]["avg_ms"]
+            kernels_transform = {"passes": 2, "ms_per_chunk": round(both, 5), "points_per_chunk": pts,
+                                 "achieved_gbs": round(16.0 * pts / (both * 1e-3) / 1e9, 1)}
+        else:
+            kernels_transform = None
         stream.synchronize()
         head = xout.download((4096,), np.complex64)  # the first 32 KiB of the output of the last execute
         per_shape.append({"n": n, "batch": batch, "ms": round(ms, 4), "gbs": round(gbs, 1),
                           "ffts_per_s": round(batch / (ms * 1e-3), 1), "plan": {k: info[k] for k in (
                               "algorithm", "n1", "n2", "e", "e1", "tile1", "e2", "tile2", "chunk")},
-                          "kernels": kernels, "output_head_fnv1a64": fnv1a64(head.tobytes())})
+                          "kernels": kernels, "transform_level": kernels_transform,
+                          "output_head_fnv1a64": fnv1a64(head.tobytes())})
     result["per_shape"] = per_shape
     k, n, batch, info = best
+    bpp = k["algorithmic_bytes_per_point"]
     result["roofline"] = {
         "bound": "hbm",
         "achieved": k["achieved_gbs"],
@@ -426,9 +487,17 @@ def main():
         "unit": "GB/s",
         "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": f"{k['phase']} of n={n} (algorithmic 16 B per complex point per launch; "
-                  f"{k['points_per_launch']} points per launch, avg {k['avg_ms']} ms, hipEvents on the launch stream)",
+        "kernel": f"{k['phase']} of n={n}: algorithmic {bpp:g} B per complex point per launch (one read or write "
+                  f"of the transform's own input/output per pass, intermediates excluded); "
+                  f"{k['points_per_launch']} points per launch, avg {k['avg_ms']} ms, hipEvents on the launch stream",
     }
+    tl = next((s_["transform_level"] for s_ in per_shape if s_["n"] == n), None)
+    if tl is not None:
+        result["roofline"]["transform_level"] = {
+            "achieved": tl["achieved_gbs"], "frac": round(tl["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "what": f"both passes of n={n} per chunk: 16 B per point over the sum of the two launches' average time"}
+    # HBM traffic per launch: not measurable inside this process (it needs two rocprofv3 --pmc runs of this same
+    # command); the figure comes from those runs' summary, and the line says so
     traffic_file = os.path.join(HERE, "profiles", "hbm_traffic.json")
     if os.path.exists(traffic_file):
         try:
@@ -436,6 +505,10 @@ def main():
             key = f"n={n} {k['phase']}"
             if key in tr:
                 result["roofline"]["traffic"] = tr[key]["bytes_per_launch"]
+                result["roofline"]["traffic_source"] = (
+                    "stored: profiles/hbm_traffic.json, rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
+                    "command (calibrated on a copy kernel), L2-miss bytes of every pass including the "
+                    "intermediate; not measured in this run")
         except (ValueError, KeyError):
             pass
 
@@ -460,6 +533,11 @@ def main():
         acc_report.append({"n": n, "transforms": m, "rel_l2_vs_fp64": rel_l2(y.astype(np.complex128), ref)})
     result["accuracy"] = acc_report
 
+    want = set(x for x in args.extras.split(",") if x)
+    unknown = want - set(EXTRAS)
+    if unknown:
+        print(f"unknown --extras {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
     if not args.no_extras:
         extras = {}
         # device copy ceiling (read + write bytes): the library's streaming copy kernel (one 16-byte element per
@@ -467,14 +545,15 @@ def main():
         from crd_amd import _native as N
 
         xin, xout = bufs[0]
-        ms = time_loop(lambda: N.check(N.lib().crdfft_stream_copy(xout.ptr, xin.ptr, xin.nbytes, stream.handle),
-                                       "crdfft_stream_copy"), stream, 5, 1, Event)
-        extras["device_copy_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
-        ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
-        extras["hip_memcpy_d2d_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
+        if "copy" in want:
+            ms = time_loop(lambda: N.check(N.lib().crdfft_stream_copy(xout.ptr, xin.ptr, xin.nbytes, stream.handle),
+                                           "crdfft_stream_copy"), stream, 5, 1, Event)
+            extras["device_copy_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
+            ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
+            extras["hip_memcpy_d2d_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
         # R2C -> C2R round trip of both shapes
         rts = []
-        for idx, (n, batch) in enumerate(SHAPES):
+        for idx, (n, batch) in enumerate(SHAPES if "roundtrip" in want else []):
             r2c = FftPlan(n, batch, "r2c")
             c2r = FftPlan(n, batch, "c2r")
             real_in = DeviceBuffer(4 * n * batch)
@@ -494,32 +573,22 @@ def main():
             rts.append({"n": n, "batch": batch, "ms": round(ms, 4), "gbs": round(bytes_rt / (ms * 1e-3) / 1e9, 1),
                         "max_abs_roundtrip_err": err})
             del r2c, c2r, real_in, spec, back
-        extras["r2c_c2r_roundtrip"] = rts
-        # n = 4096 x 1024 over 4 rotating buffers
-        n, batch, nb = 4096, 1024, 4
-        p = FftPlan(n, batch)
-        ins = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
-        outs = [DeviceBuffer(8 * n * batch) for _ in range(nb)]
-        for i, b in enumerate(ins):
-            fill_normal(b, 2 * n * batch, seed=3000 + i)
-        state = {"i": 0}
-
-        def rot():
-            i = state["i"] % nb
-            state["i"] += 1
-            p.execute(ins[i], outs[i], crd_amd.FORWARD, stream)
-
-        ms = time_loop(rot, stream, 40, 10, Event)
-        extras["c2c_n4096_b1024"] = {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
-                                     "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
-        extras["mixed_size"] = bench_mixed(stream, Event)
-        extras["split_precision"] = bench_split(stream, Event)
-        extras["spectral"] = bench_spectral(stream, Event)
-        extras["r2r_and_bluestein"] = bench_r2r(stream, Event)
+        if rts:
+            extras["r2c_c2r_roundtrip"] = rts
+        if "c2c4096" in want:
+            extras["c2c_n4096_b1024"] = bench_c2c4096(stream, Event)
+        if "mixed" in want:
+            extras["mixed_size"] = bench_mixed(stream, Event)
+        if "split" in want:
+            extras["split_precision"] = bench_split(stream, Event)
+        if "spectral" in want:
+            extras["spectral"] = bench_spectral(stream, Event)
+        if "r2r" in want:
+            extras["r2r_and_bluestein"] = bench_r2r(stream, Event)
         result["extras"] = extras
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = os.cpu_count() or 1
         log(f"cpu baseline on {threads} threads")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds, threads)
     else:

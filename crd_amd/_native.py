@@ -13,6 +13,7 @@ INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
 
 # status codes (crdfft_status)
 OK, INVALID_SIZE, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, SELF_CHECK_FAILED, UNSUPPORTED, NO_DEVICE = range(8)
+CHECK_NOT_RUN, CHECK_PASSED, CHECK_DEMOTED = range(3)  # crdfft_check_status
 FORWARD, INVERSE = 0, 1
 C2C, R2C, C2R = 0, 1, 2
 FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
@@ -57,6 +58,7 @@ class PlanInfo(ctypes.Structure):
         ("workspace_bytes", ctypes.c_uint64),
         ("table_bytes", ctypes.c_uint64),
         ("hints", ctypes.c_uint32),
+        ("check", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -166,6 +168,7 @@ _SIGS = {
     "crdfft_device_props_get": (_i32, [_i32, ctypes.POINTER(DeviceProps)]),
     "crdfft_malloc": (_i32, [ctypes.POINTER(_vp), _u64]),
     "crdfft_free": (_i32, [_vp]),
+    "crdfft_pointer_is_device": (_i32, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "crdfft_memcpy_h2d": (_i32, [_vp, _vp, _u64]),
     "crdfft_memcpy_d2h": (_i32, [_vp, _vp, _u64]),
     "crdfft_memcpy_d2d": (_i32, [_vp, _vp, _u64, _vp]),

@@ -8,10 +8,12 @@
 // so HBM sees one read and one write per point while the slot (n * 8 bytes per group) stays in the L2 of the XCD all
 // members run on. The reference's four-step is fft_kernels.cpp:2363- (phase 1 + twiddle, phase 2 + scatter).
 //
-// Coherence: a workgroup's stores reach its XCD's L2 before it counts itself in (s_waitcnt vmcnt(0), then a relaxed
-// atomic), and readers load the slot with the L1-bypassing cache policy, so members must share one L2. Groups are
-// therefore formed at launch from the XCC_ID register each workgroup reads: a per-XCD ticket, a grid-wide arrival
-// count, and a mode word decided once by compare-and-swap. When every workgroup has arrived (the launch is resident
+// Coherence, by the memory model's agent-scope release/acquire: every thread of a workgroup runs a release fence after
+// its slot stores, the workgroup barrier follows, and one thread counts the workgroup in with a release atomic; a
+// waiter polls with acquire loads, and after the barrier that releases its workgroup every thread runs an acquire
+// fence before it reads the slot. Members share one XCD (so the slot stays in that XCD's L2 and the fences' cache
+// maintenance stays local in practice): groups are formed at launch from the XCC_ID register each workgroup reads,
+// a per-XCD ticket, a grid-wide arrival count, and a mode word decided once by compare-and-swap. When every workgroup has arrived (the launch is resident
 // as a whole) each XCD's workgroups form groups of G and the remainder runs alone; when a bounded wait for the grid
 // gives up, or kHintCoopSolo is set, every workgroup runs alone on its own slot (the k_fused scheme). Every wait is a
 // bounded poll; one that gives up counts an error in the control words and lets the workgroup finish without
@@ -63,6 +65,20 @@ __device__ __forceinline__ uint32_t atom_add(uint32_t* p, uint32_t v)
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// counts this workgroup in after its stores (release: they are visible to whoever acquires the count)
+__device__ __forceinline__ void atom_add_release(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t load_acquire(uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // the value before the exchange
 __device__ __forceinline__ uint32_t atom_cas(uint32_t* p, uint32_t expected, uint32_t desired)
 {
@@ -71,14 +87,12 @@ __device__ __forceinline__ uint32_t atom_cas(uint32_t* p, uint32_t expected, uin
     return expected;
 }
 
-__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
-// Polls *p (by atomic read) until it reaches target, at most `limit` times with a short sleep between polls.
+// Polls *p (acquire loads) until it reaches target, at most `limit` times with a short sleep between polls.
 __device__ bool wait_at_least(uint32_t* p, uint32_t target, uint32_t limit)
 {
     for (uint32_t it = 0; it < limit; ++it)
     {
-        if (atom_add(p, 0u) >= target)
+        if (load_acquire(p) >= target)
         {
             return true;
         }
@@ -87,14 +101,14 @@ __device__ bool wait_at_least(uint32_t* p, uint32_t target, uint32_t limit)
     return false;
 }
 
-// v[m] = element (off + m * stride) of the resource's array, read around the L1 (the L2 holds other CUs' stores)
+// v[m] = element (off + m * stride) of the slot (plain loads: the caller has run its acquire fence)
 template <int E>
-__device__ __forceinline__ void load_l2(cf (&v)[E], __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t stride)
+__device__ __forceinline__ void load_slot(cf (&v)[E], __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t stride)
 {
 #pragma unroll
     for (int m = 0; m < E; ++m)
     {
-        v[m] = bld<1>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u);
+        v[m] = bld<0>(r, off * 8u, static_cast<uint32_t>(m) * stride * 8u);
     }
 }
 
@@ -180,8 +194,7 @@ __global__ __launch_bounds__(TH) void k_coop(CoopArgs a)
         {
             const uint32_t x = xcc_id();
             const uint32_t ticket = atom_add(ctl + kCoopCtlXcd + 16 * x, 1u);
-            wait_stores(); // the ticket is counted before the arrival
-            atom_add(ctl + kCoopCtlGrid, 1u);
+            atom_add_release(ctl + kCoopCtlGrid, 1u); // the ticket is counted before the arrival
             const bool all = wait_at_least(ctl + kCoopCtlGrid, gridDim.x, a.spin_limit);
             const uint32_t old = atom_cas(ctl + kCoopCtlMode, 0u, all ? 1u : 2u);
             const uint32_t mode = old == 0u ? (all ? 1u : 2u) : old;
@@ -242,6 +255,16 @@ __global__ __launch_bounds__(TH) void k_coop(CoopArgs a)
             s_ctl[4] = 1; // no further waits: finish the loop and exit
         }
         __syncthreads();
+        fence_acquire(); // every thread: the members' released stores are visible to its loads
+    };
+    // all threads: this workgroup's stores (or reads) of the slot are done; one thread counts it in
+    auto count_in = [&](uint32_t* p) {
+        fence_release();
+        __syncthreads();
+        if (tid == 0)
+        {
+            atom_add_release(p, 1u);
+        }
     };
     const __amdgpu_buffer_rsrc_t slot = make_rsrc(a.slots + static_cast<uint64_t>(unit) * kN);
     const LdsView<CC> va{lds + c, C::IM};
@@ -298,11 +321,13 @@ __global__ __launch_bounds__(TH) void k_coop(CoopArgs a)
             }
             store_out<E>(v, slot, a_off + ct * CC, a_stride, 1.0f, false);
         }
-        wait_stores();
-        __syncthreads();
-        if (!solo && tid == 0)
+        if (!solo)
         {
-            atom_add(a_done, 1u);
+            count_in(a_done);
+        }
+        else
+        {
+            __syncthreads();
         }
         if (t + units < a.count)
         {
@@ -317,15 +342,10 @@ __global__ __launch_bounds__(TH) void k_coop(CoopArgs a)
         for (int rt = first; rt < G; rt += step)
         {
             cf v[E];
-            load_l2<E>(v, slot, b_off + static_cast<uint32_t>(rt * RR) * N1, static_cast<uint32_t>(T));
+            load_slot<E>(v, slot, b_off + static_cast<uint32_t>(rt * RR) * N1, static_cast<uint32_t>(T));
             if (!solo)
             {
-                wait_stores();
-                __syncthreads();
-                if (tid == 0)
-                {
-                    atom_add(b_done, 1u); // this member is done reading the slot
-                }
+                count_in(b_done); // this member is done reading the slot (its loads are ordered before the count)
             }
             fft_core<N1, E, INV, 1>(v, i2, vb, tw);
             if constexpr (N1 == E)

@@ -313,6 +313,74 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// k_cols_em: one pass of the element-major four-step (strided columns, separate input and output row strides)
+// ---------------------------------------------------------------------------------------------------------------
+// Element i = i1*n2 + i2 of transform t sits at (i1*n2 + i2)*B + t. Pass 1 transforms over i1 for every (i2, t):
+// matrix i2 holds rows i1 at stride n2*B and the B transforms as adjacent columns, and its outputs k1 are twiddled
+// by W_n^{i2 k1}. Pass 2 transforms over i2 for every (k1, t): matrix k1 is contiguous (rows i2 at stride B), and
+// output k2 goes to element k1 + n1*k2, i.e. row stride n1*B from base k1*B. A workgroup owns CC adjacent columns
+// of one matrix; its loads and stores are runs of CC*8 bytes.
+template <int N1, int E, int CC, bool INV, bool TW>
+__global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols_em(EmArgs a)
+{
+    using C = ColCfg<N1, E, CC>;
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int c = tid % CC;
+    const int i = tid / CC;
+    const uint64_t groups = (a.ncols + CC - 1) / CC;
+    const uint64_t lb = blockIdx.x;
+    const uint64_t t = lb / groups;
+    const uint64_t col = (lb - t * groups) * CC + c;
+    const bool active = t < a.count && col < a.ncols;
+    cf v[E];
+    if (active)
+    {
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist + col + static_cast<uint64_t>(i) * a.in_ld;
+        const uint64_t step = static_cast<uint64_t>(C::T) * a.in_ld;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = src[static_cast<uint64_t>(m) * step];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            v[m] = cf{0.0f, 0.0f};
+        }
+    }
+    const LdsView<CC> view{lds + c, C::IM};
+    fft_core<N1, E, INV, CC>(v, i, view, reinterpret_cast<const cf*>(a.tw));
+    if (!active)
+    {
+        return;
+    }
+    if constexpr (TW)
+    {
+        const cf* thi = reinterpret_cast<const cf*>(a.thi);
+        const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+        const uint32_t lmask = (1u << a.hshift) - 1;
+        const uint32_t tt = static_cast<uint32_t>(t + a.t0); // (t0 + t) * k1 < n <= 2^24
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+        {
+            const uint32_t e = tt * static_cast<uint32_t>(i + C::T * m);
+            v[m] = ctw<INV>(v[m], cmul(thi[e >> a.hshift], tlo[e & lmask]));
+        }
+    }
+    cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist + col + static_cast<uint64_t>(i) * a.out_ld;
+    const uint64_t ostep = static_cast<uint64_t>(C::T) * a.out_ld;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+    {
+        dst[static_cast<uint64_t>(m) * ostep] = v[m];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // k_cols_pf: the column pass as a persistent launch that loads the next tile while it transforms this one
 // ---------------------------------------------------------------------------------------------------------------
 // For long columns (n1 >= 1024) one k_cols workgroup fills a CU's LDS, so nothing overlaps its load phase with its
@@ -891,6 +959,28 @@ __global__ void k_ref_bitrev(const cf* in, cf* out, uint64_t n, uint32_t lg, uin
     out[t * odist + r] = in[t * idist + i];
 }
 
+// The in-place bit reversal (fft.hpp:421-431 swaps data[i] and data[rev(i)] for i < rev(i)): the thread of the
+// smaller index of each pair swaps it, so every element is read and written by one thread only.
+__global__ void k_ref_bitrev_inplace(cf* data, uint64_t n, uint32_t lg, uint64_t count, uint64_t dist)
+{
+    const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n * count)
+    {
+        return;
+    }
+    const uint64_t t = idx / n;
+    const uint64_t i = idx - t * n;
+    const uint64_t r = lg == 0 ? 0 : (__brevll(i) >> (64 - lg));
+    if (i < r)
+    {
+        cf* base = data + t * dist;
+        const cf a = base[i];
+        const cf b = base[r];
+        base[i] = b;
+        base[r] = a;
+    }
+}
+
 template <bool INV>
 __global__ void k_ref_stage(cf* data, const cf* w, uint64_t n, uint64_t len, uint64_t count, uint64_t dist)
 {
@@ -1110,6 +1200,56 @@ hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a
         return hipGetLastError();                                                                                      \
     }
     CRDFFT_COL_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// (N1, E, CC) of the element-major four-step's column kernels: every length a split of 2^13..2^24 gives
+#define CRDFFT_COL_EM_LIST(X)                                                                                          \
+    X(64, 16, 16) X(128, 16, 16) X(256, 16, 16) X(512, 16, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
+
+bool em_col_shape(uint64_t len, int* e, int* c)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (len == N)                                                                                                      \
+    {                                                                                                                  \
+        *e = E;                                                                                                        \
+        *c = CC;                                                                                                       \
+        return true;                                                                                                   \
+    }
+    CRDFFT_COL_EM_LIST(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_cols_em(uint64_t len, bool inverse, const EmArgs& a, hipStream_t s)
+{
+#define X(N, E, CC)                                                                                                    \
+    if (len == N)                                                                                                      \
+    {                                                                                                                  \
+        using C = ColCfg<N, E, CC>;                                                                                    \
+        const bool tw = a.thi != nullptr;                                                                              \
+        auto kern = inverse ? (tw ? k_cols_em<N, E, CC, true, true> : k_cols_em<N, E, CC, true, false>)                \
+                            : (tw ? k_cols_em<N, E, CC, false, true> : k_cols_em<N, E, CC, false, false>);             \
+        hipError_t err = set_lds(kern, C::lds_bytes);                                                                  \
+        if (err != hipSuccess)                                                                                         \
+            return err;                                                                                                \
+        const uint64_t groups_per = (a.ncols + CC - 1) / CC;                                                           \
+        const uint64_t max_t = std::max<uint64_t>(1, (kMaxItems / C::threads) / groups_per);                          \
+        for (uint64_t done = 0; done < a.count;)                                                                       \
+        {                                                                                                              \
+            EmArgs part = a;                                                                                           \
+            part.count = std::min<uint64_t>(a.count - done, max_t);                                                    \
+            part.in = a.in + done * a.idist;                                                                           \
+            part.out = a.out + done * a.odist;                                                                         \
+            part.t0 = a.t0 + done;                                                                                     \
+            const uint64_t groups = groups_per * part.count;                                                           \
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(groups)), dim3(C::threads), C::lds_bytes, s, part);   \
+            done += part.count;                                                                                        \
+        }                                                                                                              \
+        return hipGetLastError();                                                                                      \
+    }
+    CRDFFT_COL_EM_LIST(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1443,9 +1583,9 @@ hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint
     {
         return hipSuccess;
     }
-    if (in == out)
+    if (in == out && idist != odist)
     {
-        return hipErrorInvalidValue; // the bit reversal is out of place
+        return hipErrorInvalidValue; // in place needs one layout
     }
     const uint32_t lg = static_cast<uint32_t>(ilog2c(static_cast<long>(n)));
     const uint64_t items = n * count;
@@ -1453,8 +1593,16 @@ hipError_t launch_reference(const cfloat* in, cfloat* out, const cfloat* w, uint
     {
         return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_ref_bitrev, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<const cf*>(in), reinterpret_cast<cf*>(out), n, lg, count, idist, odist);
+    if (in == out)
+    {
+        hipLaunchKernelGGL(k_ref_bitrev_inplace, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<cf*>(out), n, lg, count, odist);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_ref_bitrev, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<const cf*>(in), reinterpret_cast<cf*>(out), n, lg, count, idist, odist);
+    }
     const uint64_t bfly = (n / 2) * count;
     for (uint64_t len = 2; len <= n; len <<= 1)
     {

@@ -146,6 +146,31 @@ hipError_t launch_rows(uint64_t n, int e, bool inverse, const RowArgs& a, hipStr
 hipError_t launch_fused(uint64_t n1, int e1, uint64_t n2, int e2, int threads, bool inverse, const FusedArgs& a,
                         uint64_t grid, hipStream_t s);
 hipError_t launch_cols(uint64_t n1, int e, int c, bool inverse, const ColArgs& a, hipStream_t s);
+
+// Element-major batches of n = n1 * n2 > 4096 (data[i*batch + t], fft.hpp:462-470): the four-step over strided
+// columns. Each pass transforms `ncols` adjacent columns of `count` matrices, matrix t at in + t*idist with rows
+// in_ld apart, and writes them to out + t*odist with rows out_ld apart (64-bit offsets throughout). With thi set,
+// the pass multiplies value k1 of matrix t by W_n^{t*k1} = thi[e >> hshift] * tlo[e & (2^hshift - 1)],
+// e = (t0 + t)*k1.
+struct EmArgs
+{
+    const cfloat* in;
+    cfloat* out;
+    const cfloat* tw; // stage twiddles of the column length
+    const cfloat* thi;
+    const cfloat* tlo;
+    uint32_t hshift;
+    uint64_t ncols;
+    uint64_t count;
+    uint64_t idist;
+    uint64_t odist;
+    uint64_t in_ld;
+    uint64_t out_ld;
+    uint64_t t0 = 0; // matrix index of the first matrix (the twiddle exponent counts from it)
+};
+// (values per thread, columns per workgroup) of the element-major column kernel of length len, or false.
+bool em_col_shape(uint64_t len, int* e, int* c);
+hipError_t launch_cols_em(uint64_t len, bool inverse, const EmArgs& a, hipStream_t s);
 // The inverse column pass of a C2R plan's half-length four-step with the pre-rotation fused into its loads.
 bool have_cols_c2r(uint64_t n1, int e, int c);
 hipError_t launch_cols_c2r(uint64_t n1, int e, int c, const ColArgs& a, hipStream_t s);
@@ -201,6 +226,8 @@ hipError_t launch_blue_fused(uint64_t m, int e, bool inverse, const BlueArgs& a,
 hipError_t launch_transpose(const cfloat* in, cfloat* out, uint64_t rows, uint64_t cols, uint64_t count,
                             uint64_t idist, uint64_t odist, hipStream_t s);
 hipError_t launch_scale(cfloat* data, uint64_t count, float scale, hipStream_t s);
+// data[t*dist + i] *= scale for i < n, t < count.
+hipError_t launch_scale_rows(cfloat* data, uint64_t n, uint64_t count, uint64_t dist, float scale, hipStream_t s);
 
 // The radix-2 reference transform (bit reversal + log2 n butterfly stages over global memory) of `count`
 // contiguous transforms; `w` holds W_n^k for k < n/2.

@@ -64,7 +64,32 @@ __global__ void k_scale(cf* data, uint64_t count, float s)
     }
 }
 
+// data[t*dist + i] *= s for i < n, t < count (the rows of a padded batch; the padding is left alone)
+__global__ void k_scale_rows(cf* data, uint64_t n, uint64_t count, uint64_t dist, float s)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * count; idx += stride)
+    {
+        const uint64_t t = idx / n;
+        cf* p = data + t * dist + (idx - t * n);
+        const cf v = *p;
+        *p = cf{v.x * s, v.y * s};
+    }
+}
+
 } // namespace
+
+hipError_t launch_scale_rows(cfloat* data, uint64_t n, uint64_t count, uint64_t dist, float scale, hipStream_t s)
+{
+    if (n * count == 0)
+    {
+        return hipSuccess;
+    }
+    const uint64_t blocks = std::min<uint64_t>((n * count + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_scale_rows, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                       reinterpret_cast<cf*>(data), n, count, dist, scale);
+    return hipGetLastError();
+}
 
 hipError_t launch_transpose(const cfloat* in, cfloat* out, uint64_t rows, uint64_t cols, uint64_t count,
                             uint64_t idist, uint64_t odist, hipStream_t s)

@@ -3,6 +3,7 @@
 // Host code only (C++20, compiled by hipcc for its HIP runtime calls); every transform runs in the HIP kernels of
 // fft_kernels.hip and real_kernels.hip. There is no CPU fallback: a plan that cannot be built on the device is an
 // error returned to the caller.
+#include <bit>
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -142,6 +143,11 @@ struct crdfft_plan_s
     DeviceBuffer wref;      // W_cn^k, k < cn/2, for the radix-2 reference (built on first use)
     DeviceBuffer workspace; // four-step intermediate, chunk * complex_n
     DeviceBuffer tw_em;     // stage table of the element-major kernel (built on first use)
+    // element-major batches of n > 4096 (built on first use): stage tables of both column lengths, the factored
+    // W_n, and a grow-only workspace per stream
+    DeviceBuffer em_tw1, em_tw2, em_thi, em_tlo;
+    uint32_t em_hshift = 0;
+    std::vector<std::pair<hipStream_t, std::unique_ptr<DeviceBuffer>>> em_work;
     DeviceBuffer split_fwd, split_inv, tw256; // split-precision MFMA passes: operand splits, W_256
     // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
     crdfft_plan inner = nullptr;
@@ -150,6 +156,8 @@ struct crdfft_plan_s
     uint64_t blue_chunk = 0;
     uint64_t coop_ctl_offset = 0; // COOP: byte offset of the control words in the workspace
     int blue_e = 0; // > 0: the one-kernel form at this many values per thread (its stage table in tw_single)
+    bool floor = false;           // demoted: every transform runs the radix-2 reference kernels (crdfft_check_status)
+    int32_t check = CRDFFT_CHECK_NOT_RUN;
     PerStream workspace_of, bwork_of; // per-stream copies of workspace / bwork
     AuxPipes pipes;                   // per caller stream: the second pipeline of a two-pipeline four-step
     std::mutex ref_mutex;
@@ -362,6 +370,19 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
 {
     const crdfft::PlanShape& sh = p->shape;
     const uint64_t cn = sh.complex_n;
+    if (p->floor && cn >= 2)
+    {
+        // the demotion floor: the radix-2 reference kernels over device memory (in place when in == out), then the
+        // plan's scale on the rows only (the padding of a padded layout is the caller's)
+        CRDFFT_TRY(prof.before());
+        CRDFFT_TRY(crdfft::launch_reference(in, out, p->wref.as<cfloat>(), cn, count, idist, odist, inverse, s));
+        if (scale != 1.0f)
+        {
+            CRDFFT_TRY(crdfft::launch_scale_rows(out, cn, count, odist, scale, s));
+        }
+        CRDFFT_TRY(prof.after(0, cn * count));
+        return CRDFFT_OK;
+    }
     switch (sh.algorithm)
     {
         case CRDFFT_ALGO_COPY:
@@ -540,7 +561,7 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
     {
         return CRDFFT_OK;
     }
-    if (sh.precision != CRDFFT_FP32)
+    if (sh.precision != CRDFFT_FP32 && !p->floor)
     {
         // split-precision MFMA four-step of 256 x 256: column pass into the workspace, row pass out
         const cfloat* x = static_cast<const cfloat*>(in);
@@ -649,7 +670,8 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
             cfloat* o = static_cast<cfloat*>(out);
             int se = 0;
             unsigned per_group = 0;
-            if (sh.algorithm == CRDFFT_ALGO_SINGLE && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && h >= 256 &&
+            if (sh.algorithm == CRDFFT_ALGO_SINGLE && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && !p->floor &&
+                h >= 256 &&
                 crdfft::spec_shape(h, &se, &per_group) && se == sh.e)
             {
                 // 512 <= n <= 32768: load, half-length core, recombine and store in one launch (the spectral
@@ -662,7 +684,7 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
                 CRDFFT_TRY(prof.after(0, h * sh.batch));
                 return CRDFFT_OK;
             }
-            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 &&
+            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && !p->floor &&
                 (sh.hints & crdfft::kHintPersistent) == 0 && crdfft::have_rowtr_r2c(sh.n2, sh.e2, sh.tile2) &&
                 sh.n1 >= static_cast<uint64_t>(sh.tile2))
             {
@@ -689,7 +711,8 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
             }
             const uint64_t h = sh.n / 2;
             cfloat* o = static_cast<cfloat*>(out);
-            if (sh.algorithm == CRDFFT_ALGO_SINGLE && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && h >= 512 &&
+            if (sh.algorithm == CRDFFT_ALGO_SINGLE && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && !p->floor &&
+                h >= 512 &&
                 crdfft::c2r_one_kernel_e(h) == sh.e)
             {
                 // 1024 <= n <= 32768: recombine, inverse half-length core and store in one launch (1.1-1.4x the
@@ -702,7 +725,7 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
                 CRDFFT_TRY(prof.after(0, h * sh.batch));
                 return CRDFFT_OK;
             }
-            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 &&
+            if (sh.algorithm == CRDFFT_ALGO_FOUR_STEP && (p->flags & CRDFFT_FLAG_SEPARATE_PASSES) == 0 && !p->floor &&
                 (sh.hints & crdfft::kHintPersistent) == 0 && crdfft::have_cols_c2r(sh.n1, sh.e1, sh.tile1) &&
                 sh.n2 % static_cast<uint64_t>(sh.tile1) == 0)
             {
@@ -776,7 +799,28 @@ crdfft_status create_plan(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_
     }
     if (rs == CRDFFT_OK && (flags & CRDFFT_FLAG_NO_SELF_CHECK) == 0)
     {
+        // select_checked_kernel_level (kernel_dispatch.hpp:105-145): the selected kernels, else the floor
         rs = crdfft_self_check(p);
+        if (rs == CRDFFT_OK)
+        {
+            p->check = CRDFFT_CHECK_PASSED;
+        }
+        else if (rs == CRDFFT_SELF_CHECK_FAILED && shape.complex_n >= 2)
+        {
+            rs = crdfft::plan_use_floor(p);
+            if (rs == CRDFFT_OK)
+            {
+                rs = hip_status(hipStreamSynchronize(nullptr));
+            }
+            if (rs == CRDFFT_OK)
+            {
+                rs = crdfft_self_check(p);
+            }
+            if (rs == CRDFFT_OK)
+            {
+                p->check = CRDFFT_CHECK_DEMOTED;
+            }
+        }
     }
     if (rs != CRDFFT_OK)
     {
@@ -999,6 +1043,109 @@ crdfft_status crdfft_execute_profiled(crdfft_plan p, const void* in, void* out, 
     return hip_status(e);
 }
 
+} // extern "C"
+
+namespace crdfft
+{
+crdfft_status plan_use_floor(crdfft_plan p)
+{
+    if (p == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    const uint64_t cn = p->shape.complex_n;
+    {
+        std::lock_guard<std::mutex> lock(p->ref_mutex);
+        if (p->wref.ptr == nullptr && cn >= 2)
+        {
+            CRDFFT_TRY(p->wref.alloc((cn / 2) * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_roots(p->wref.as<cfloat>(), cn / 2, 1, cn, nullptr));
+        }
+    }
+    p->floor = true;
+    return CRDFFT_OK;
+}
+
+bool plan_on_floor(crdfft_plan p) { return p != nullptr && p->floor; }
+} // namespace crdfft
+
+namespace
+{
+// Element-major batches of n = n1 * n2 > 4096 (the reference's execute_batched takes every power of two,
+// fft_kernels.cpp:593-784): pass 1 transforms the n1-point strided columns of every (i2, t) into the stream's
+// workspace with the twiddle W_n^{i2 k1}, pass 2 the n2-point columns of every (k1, t) back into data at
+// element k1 + n1*k2. The workspace holds the whole batch (n * batch points) and grows with it.
+crdfft_status execute_batched_four_step(crdfft_plan p, cfloat* data, uint64_t batch, bool inverse, hipStream_t s)
+{
+    const uint64_t n = p->shape.complex_n;
+    const uint32_t lg = static_cast<uint32_t>(std::countr_zero(n));
+    const uint64_t n1 = uint64_t{1} << ((lg + 1) / 2);
+    const uint64_t n2 = n >> ((lg + 1) / 2);
+    int e1 = 0, c1 = 0, e2 = 0, c2 = 0;
+    if (!crdfft::em_col_shape(n1, &e1, &c1) || !crdfft::em_col_shape(n2, &e2, &c2))
+    {
+        return CRDFFT_UNSUPPORTED;
+    }
+    if (batch > (uint64_t{1} << 40) / n)
+    {
+        return CRDFFT_INVALID_ARGUMENT; // more than 8 TiB of data
+    }
+    const uint64_t bytes = n * batch * sizeof(cfloat);
+    cfloat* ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(p->ref_mutex);
+        if (p->em_thi.ptr == nullptr)
+        {
+            const uint64_t t1 = crdfft::stage_twiddle_entries(n1, e1), t2 = crdfft::stage_twiddle_entries(n2, e2);
+            CRDFFT_TRY(p->em_tw1.alloc(std::max<uint64_t>(t1, 1) * sizeof(cfloat)));
+            CRDFFT_TRY(p->em_tw2.alloc(std::max<uint64_t>(t2, 1) * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_stage_twiddles(p->em_tw1.as<cfloat>(), n1, e1, s));
+            CRDFFT_TRY(crdfft::build_stage_twiddles(p->em_tw2.as<cfloat>(), n2, e2, s));
+            const uint32_t hshift = lg / 2;
+            const uint64_t lo = uint64_t{1} << hshift, hi = n >> hshift;
+            CRDFFT_TRY(p->em_tlo.alloc(lo * sizeof(cfloat)));
+            CRDFFT_TRY(p->em_thi.alloc(hi * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_roots(p->em_thi.as<cfloat>(), hi, lo, n, s));
+            CRDFFT_TRY(crdfft::build_roots(p->em_tlo.as<cfloat>(), lo, 1, n, s));
+            p->em_hshift = hshift;
+        }
+        std::unique_ptr<DeviceBuffer>* slot = nullptr;
+        for (auto& w : p->em_work)
+        {
+            if (w.first == s)
+            {
+                slot = &w.second;
+            }
+        }
+        if (slot == nullptr)
+        {
+            if (p->em_work.size() >= crdfft::PerStream::kMaxStreams)
+            {
+                return CRDFFT_OUT_OF_MEMORY;
+            }
+            p->em_work.emplace_back(s, std::make_unique<DeviceBuffer>());
+            slot = &p->em_work.back().second;
+        }
+        if ((*slot)->bytes < bytes)
+        {
+            auto grown = std::make_unique<DeviceBuffer>();
+            CRDFFT_TRY(grown->alloc(bytes)); // the old buffer's free (after this) waits for the device
+            *slot = std::move(grown);
+        }
+        ws = (*slot)->as<cfloat>();
+    }
+    crdfft::EmArgs a1{data, ws, p->em_tw1.as<cfloat>(), p->em_thi.as<cfloat>(), p->em_tlo.as<cfloat>(), p->em_hshift,
+                      batch, n2, batch, batch, n2 * batch, n2 * batch};
+    CRDFFT_TRY(crdfft::launch_cols_em(n1, inverse, a1, s));
+    crdfft::EmArgs a2{ws, data, p->em_tw2.as<cfloat>(), nullptr, nullptr, 0, batch, n1, n2 * batch, batch, batch,
+                      n1 * batch};
+    CRDFFT_TRY(crdfft::launch_cols_em(n2, inverse, a2, s));
+    return CRDFFT_OK;
+}
+} // namespace
+
+extern "C" {
+
 crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, int32_t direction, void* stream)
 {
     if (p == nullptr || data == nullptr || (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
@@ -1019,13 +1166,13 @@ crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, 
         return CRDFFT_OK;
     }
     int e = 0, c = 0;
-    if (!crdfft::element_major_shape(sh.complex_n, batch, &e, &c))
-    {
-        return CRDFFT_UNSUPPORTED;
-    }
-    // the element-major kernel's stage table: the plan's own when the shapes agree, else built once and kept
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool inverse = direction == CRDFFT_INVERSE;
+    if (!crdfft::element_major_shape(sh.complex_n, batch, &e, &c))
+    {
+        return execute_batched_four_step(p, static_cast<cfloat*>(data), batch, inverse, s);
+    }
+    // the element-major kernel's stage table: the plan's own when the shapes agree, else built once and kept
     const cfloat* tw = nullptr;
     if (sh.algorithm == CRDFFT_ALGO_SINGLE && sh.e == e)
     {
@@ -1050,7 +1197,7 @@ crdfft_status crdfft_execute_batched(crdfft_plan p, void* data, uint64_t batch, 
 
 crdfft_status crdfft_execute_reference(crdfft_plan p, const void* in, void* out, int32_t direction, void* stream)
 {
-    if (p == nullptr || in == nullptr || out == nullptr || in == out ||
+    if (p == nullptr || in == nullptr || out == nullptr || (in == out && p->idist != p->odist) ||
         (direction != CRDFFT_FORWARD && direction != CRDFFT_INVERSE))
     {
         return CRDFFT_INVALID_ARGUMENT;
@@ -1083,6 +1230,7 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
         return CRDFFT_INVALID_ARGUMENT;
     }
     crdfft::shape_to_info(p->shape, info);
+    info->check = p->check;
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
                         p->tstep.bytes + p->tfull.bytes + p->wreal.bytes + p->split_fwd.bytes + p->split_inv.bytes + p->tw256.bytes;
     return CRDFFT_OK;
@@ -1183,6 +1331,27 @@ crdfft_status crdfft_malloc(void** ptr, uint64_t bytes)
     }
     *ptr = nullptr;
     CRDFFT_TRY(hipMalloc(ptr, bytes));
+    return CRDFFT_OK;
+}
+
+crdfft_status crdfft_pointer_is_device(const void* ptr, int32_t* is_device)
+{
+    if (is_device == nullptr)
+    {
+        return CRDFFT_INVALID_ARGUMENT;
+    }
+    *is_device = 0;
+    if (ptr == nullptr)
+    {
+        return CRDFFT_OK;
+    }
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess)
+    {
+        (void)hipGetLastError(); // pageable host memory the runtime has never seen: not device memory
+        return CRDFFT_OK;
+    }
+    *is_device = (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) ? 1 : 0;
     return CRDFFT_OK;
 }
 

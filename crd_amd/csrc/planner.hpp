@@ -83,4 +83,9 @@ bool have_fused_kernel(uint64_t n1, int e1, uint64_t n2, int e2, int threads, un
 bool coop_shape_of(uint64_t n, uint64_t* n1, int* e, int* threads, unsigned* lds_bytes);
 constexpr uint32_t kCoopSpinLimit = 1u << 18;
 
+// The demotion floor of a plan (the radix-2 reference kernels, crdfft_check_status): switch the plan onto it
+// (building the reference's W_n table), and whether it runs there.
+crdfft_status plan_use_floor(crdfft_plan p);
+bool plan_on_floor(crdfft_plan p);
+
 } // namespace crdfft

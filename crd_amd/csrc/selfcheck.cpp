@@ -6,9 +6,10 @@
 // implementation's bits, so the answers here are analytic: an impulse at j = 1 transforms to W_n^k, a tone
 // exp(+2 pi i 3 j / n) to n at bin 3 and zero elsewhere, a constant to n at bin 0; for real plans a cosine of
 // frequency 1 to n/2 at bin 1 and back. Tolerance: 2e-6 * log2(n) * (1 + max |X|), the plain path's accuracy class.
-// A failing check returns CRDFFT_SELF_CHECK_FAILED: the plan is not created (the library has no lower level to
-// demote to). CRDFFT_FAULT_SELF_CHECK=1 in the environment flips one output bit before the comparison, the host
-// fault seam (reference crd/core/fault_seam.hpp:139) that lets a test force the failure path.
+// A failing check returns CRDFFT_SELF_CHECK_FAILED; plan creation then demotes the plan to its floor (the radix-2
+// reference kernels) and checks again (crdfft_check_status). CRDFFT_FAULT_SELF_CHECK in the environment corrupts one
+// output value before the comparison, the host fault seam (reference crd/core/fault_seam.hpp:139) that lets a test
+// force either path: "1" for the selected kernels only, "2" for every check.
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -39,11 +40,20 @@ struct DevBuf
     }
 };
 
-bool fault_forced()
+// The fault seam: "1" fails the check of a plan's selected kernels (so it demotes to its floor), "2" fails every
+// check (the floor's too).
+bool fault_forced(crdfft_plan plan)
 {
     const char* v = std::getenv("CRDFFT_FAULT_SELF_CHECK");
-    return v != nullptr && v[0] == '1';
+    if (v == nullptr)
+    {
+        return false;
+    }
+    return v[0] == '2' || (v[0] == '1' && !crdfft::plan_on_floor(plan));
 }
+
+// The single-transform plan the check runs: the same kernels as `plan`, on its floor when the plan is.
+crdfft_status check_plan(crdfft_plan plan, const crdfft_plan_info& info, crdfft_plan* one);
 
 // Indices checked for an n-point output: all of them up to 2^16, else a stride that visits 2^16 of them plus the
 // first and last 64.
@@ -71,6 +81,23 @@ crdfft_tuning same_kernels(const crdfft_plan_info& info)
     return t;
 }
 
+crdfft_status check_plan(crdfft_plan plan, const crdfft_plan_info& info, crdfft_plan* one)
+{
+    const crdfft_tuning same = same_kernels(info);
+    crdfft_status st = crdfft_plan_create_tuned(one, info.n, 1, 0, 0, info.type, CRDFFT_FP32,
+                                                CRDFFT_FLAG_NO_SELF_CHECK, &same);
+    if (st == CRDFFT_OK && crdfft::plan_on_floor(plan))
+    {
+        st = crdfft::plan_use_floor(*one);
+        if (st != CRDFFT_OK)
+        {
+            crdfft_plan_destroy(*one);
+            *one = nullptr;
+        }
+    }
+    return st;
+}
+
 } // namespace
 
 extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
@@ -92,7 +119,7 @@ extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
     const uint64_t n = info.n;
     const double lg = std::max(1.0, std::log2(static_cast<double>(n)));
     const double tol = 2e-6 * lg;
-    const bool fault = fault_forced();
+    const bool fault = fault_forced(plan);
 
     if (info.type == CRDFFT_C2C)
     {
@@ -108,8 +135,7 @@ extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
         // a single-transform plan with the same kernels as `plan` (same split, values per thread and tiles; only
         // the chunk differs, which does not change the arithmetic)
         crdfft_plan one = nullptr;
-        const crdfft_tuning same = same_kernels(info);
-        st = crdfft_plan_create_tuned(&one, n, 1, 0, 0, CRDFFT_C2C, CRDFFT_FP32, CRDFFT_FLAG_NO_SELF_CHECK, &same);
+        st = check_plan(plan, info, &one);
         if (st != CRDFFT_OK)
         {
             return st;
@@ -207,8 +233,7 @@ extern "C" crdfft_status crdfft_self_check(crdfft_plan plan)
         return CRDFFT_OUT_OF_MEMORY;
     }
     crdfft_plan one = nullptr;
-    const crdfft_tuning same = same_kernels(info);
-    st = crdfft_plan_create_tuned(&one, n, 1, 0, 0, info.type, CRDFFT_FP32, CRDFFT_FLAG_NO_SELF_CHECK, &same);
+    st = check_plan(plan, info, &one);
     if (st != CRDFFT_OK)
     {
         return st;

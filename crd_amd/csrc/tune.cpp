@@ -242,6 +242,17 @@ crdfft_status crdfft_time_plan(uint64_t n, uint64_t batch, int32_t type, const c
         {
             st = CRDFFT_HIP_ERROR;
         }
+        if (st == CRDFFT_OK && tuning != nullptr && tuning->algorithm == CRDFFT_ALGO_COOP)
+        {
+            // a cooperative launch whose bounded group wait gave up skipped work: its time is not a plan's time,
+            // and the candidate is dropped (crdfft_autotune_plan skips SELF_CHECK_FAILED candidates)
+            uint32_t given_up = 0, mode = 0;
+            st = crdfft_plan_coop_status(plan, s, &given_up, &mode);
+            if (st == CRDFFT_OK && given_up != 0)
+            {
+                st = CRDFFT_SELF_CHECK_FAILED;
+            }
+        }
     }
     crdfft_plan_info info{};
     if (st == CRDFFT_OK)
@@ -296,9 +307,9 @@ crdfft_status crdfft_autotune_plan(uint64_t n, uint64_t batch, int32_t type, uin
     {
         crdfft_plan_record rec{};
         const crdfft_status st = crdfft_time_plan(n, batch, type, &t, reps, &rec);
-        if (st == CRDFFT_OUT_OF_MEMORY)
+        if (st == CRDFFT_OUT_OF_MEMORY || st == CRDFFT_SELF_CHECK_FAILED)
         {
-            continue; // a candidate whose workspace does not fit is not a candidate
+            continue; // a candidate whose workspace does not fit, or whose cooperative waits gave up, is not one
         }
         if (st != CRDFFT_OK)
         {

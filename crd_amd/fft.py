@@ -4,7 +4,7 @@ Names, argument meaning and error behaviour follow engine/numerics/hesap-fft/inc
   FftPlan(n)                  fft.hpp:220-304  (a device plan; `batch` transforms per execute)
   FftPlan.execute             fft.hpp:409-413  (device buffers; in place when out is in)
   FftPlan.execute_batched     fft.hpp:462-470  (element-major, in place)
-  FftPlan.execute_reference   fft.hpp:415-460  (radix-2 reference, out of place)
+  FftPlan.execute_reference   fft.hpp:415-460  (radix-2 reference, in place or out of place)
   FftPlan.size                fft.hpp:405
   RealFftPlan.rfft / .irfft   real_fft.hpp:41-93 (irfft scales by 1/n)
   fft(), ifft_normalized()    fft.hpp:1005-1029 (host arrays in, host arrays out)
@@ -167,13 +167,14 @@ class FftPlan:
     """A device plan of `batch` length-n transforms (contiguous at distance idist/odist, 0 = packed)."""
 
     def __init__(self, n, batch=1, type="c2c", precision="fp32", idist=0, odist=0, self_check=True, tuning=None,
-                 passes=False):
-        """passes=True (R2C/C2R): the separate recombine pass even where the one-kernel form applies."""
+                 passes=False, flags=0):
+        """passes=True (R2C/C2R): the separate recombine pass even where the one-kernel form applies; flags: more
+        crdfft plan flags (CRDFFT_FLAG_NORMALIZE_INVERSE)."""
         self.n = int(n)
         self.batch = int(batch)
         self.type = type
         self.handle = ctypes.c_void_p()
-        flags = N.FLAG_NONE if self_check else N.FLAG_NO_SELF_CHECK
+        flags = int(flags) | (N.FLAG_NONE if self_check else N.FLAG_NO_SELF_CHECK)
         if passes:
             flags |= N.FLAG_SEPARATE_PASSES
         if tuning is None:
@@ -210,8 +211,10 @@ class FftPlan:
         check(N.lib().crdfft_execute_batched(self.handle, _ptr(data), int(batch), int(direction), _stream(stream)),
               "crdfft_execute_batched")
 
-    def execute_reference(self, data_in, data_out, direction=N.FORWARD, stream=None):
-        check(N.lib().crdfft_execute_reference(self.handle, _ptr(data_in), _ptr(data_out), int(direction),
+    def execute_reference(self, data_in, data_out=None, direction=N.FORWARD, stream=None):
+        """The radix-2 reference transform (fft.hpp:415-460); data_out None runs it in place."""
+        out = data_in if data_out is None else data_out
+        check(N.lib().crdfft_execute_reference(self.handle, _ptr(data_in), _ptr(out), int(direction),
                                                _stream(stream)), "crdfft_execute_reference")
 
     def self_check(self):

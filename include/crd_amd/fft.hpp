@@ -5,14 +5,22 @@
 //
 // Same names, argument meaning and conventions as the reference: forward and inverse unnormalized, irfft scaled by
 // 1/n, Complex<T> interleaved {re, im}, execute in place on n values, execute_batched element-major
-// (data[i*batch + t]). Differences, all forced by the device boundary:
-//   - data lives in device memory (DeviceSpan), and calls are asynchronous on the plan's HIP stream;
-//   - a plan covers `batch` contiguous transforms (the reference loops over transforms and jobs, batch_arms.hpp);
-//   - the allocator is the device's (the reference takes an IAllocator*; a device plan owns hipMalloc'd tables);
-//   - a contract violation throws FftError carrying the crdfft status (the reference asserts, fft.hpp:237).
+// (data[i*batch + t]). Two ways in:
+//   - the reference's own call shapes, unchanged: FftPlan<f32>(alloc, n), FftPlan<f32>(alloc, n, FftAlgorithm),
+//     RealFftPlan / BluesteinPlan / DctPlan (alloc, n), NdFftPlan(alloc, dims), fft(alloc, span, dir),
+//     ifft_normalized(alloc, span), execute / execute_batched / execute_reference on any span type with data() and
+//     size() (crd::containers::Span included). A span over host memory is staged through device memory and the
+//     call returns with the result in place, like the reference's synchronous CPU call; a span over device memory
+//     runs asynchronously on the plan's stream. The allocator is accepted and not used: a device plan's tables and
+//     workspace are device memory (the reference allocates its host tables from it, fft.hpp:478-491);
+//   - the device-native forms: DeviceSpan views, a plan of `batch` contiguous transforms and a HIP stream.
+// Differences, all forced by the device boundary:
+//   - a contract violation throws FftError carrying the crdfft status (the reference asserts, fft.hpp:237);
+//   - the FftAlgorithm hint names CPU engines; algorithm_tuning() maps each onto a device plan kind.
 // Only T = float exists: the GPU path is FP32.
 #pragma once
 
+#include <concepts>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -31,6 +39,39 @@ enum class FftDirection : std::uint8_t
     Forward = 0,
     Inverse = 1,
 };
+
+// FftAlgorithm (fft_kernels.hpp:28-39): the engine a reference plan runs forward. The device plans have their own
+// kinds (crdfft_algorithm); algorithm_tuning() below records how each hint maps onto them.
+enum class FftAlgorithm : std::uint8_t
+{
+    Stockham,
+    StandaloneHier,
+    Ip4,
+    DeepSplit3,
+    DeepSplit4,
+    FourStep,
+    SixStep,
+    TiledFourStep,
+    InPlace3,
+};
+
+// fft_algorithm_name (fft_kernels.hpp:41-64): the enumerator's name.
+[[nodiscard]] constexpr const char* fft_algorithm_name(FftAlgorithm algorithm) noexcept
+{
+    switch (algorithm)
+    {
+        case FftAlgorithm::Stockham: return "Stockham";
+        case FftAlgorithm::StandaloneHier: return "StandaloneHier";
+        case FftAlgorithm::Ip4: return "Ip4";
+        case FftAlgorithm::DeepSplit3: return "DeepSplit3";
+        case FftAlgorithm::DeepSplit4: return "DeepSplit4";
+        case FftAlgorithm::FourStep: return "FourStep";
+        case FftAlgorithm::SixStep: return "SixStep";
+        case FftAlgorithm::TiledFourStep: return "TiledFourStep";
+        case FftAlgorithm::InPlace3: return "InPlace3";
+    }
+    return "?";
+}
 
 // Layout of crd::hesap::Complex<T> (engine/numerics/hesap/include/crd/hesap/complex.hpp:27-35).
 template <typename T> struct Complex
@@ -111,11 +152,132 @@ private:
     std::size_t m_size = 0;
 };
 
+// Any contiguous view with data() and size() (crd::containers::Span / ConstSpan, std::span, DeviceSpan) whose
+// elements are `Bytes` bytes: 8 for complex FP32, 4 for real FP32.
+template <typename S, std::size_t Bytes>
+concept SpanOf = requires(const S& s) {
+    { s.data() };
+    { s.size() } -> std::convertible_to<std::size_t>;
+} && sizeof(*std::declval<const S&>().data()) == Bytes;
+
+namespace detail
+{
+inline bool on_device(const void* p)
+{
+    std::int32_t dev = 0;
+    check(crdfft_pointer_is_device(p, &dev), "crdfft_pointer_is_device");
+    return dev != 0;
+}
+
+// The memory of a caller's span as the kernels see it: the span itself when it is device memory; else a device copy
+// (filled from the host when `in`), which finish() synchronizes the stream for and writes back (when `out`). The
+// host path is synchronous, as the reference's CPU call is; the device path stays asynchronous on the stream.
+class Staged
+{
+public:
+    Staged(const void* p, std::size_t bytes, bool in, bool out, void* stream)
+        : m_user(const_cast<void*>(p)), m_bytes(bytes), m_out(out), m_stream(stream)
+    {
+        if (p == nullptr || bytes == 0 || on_device(p))
+        {
+            m_dev = m_user;
+            return;
+        }
+        check(crdfft_malloc(&m_dev, bytes), "crdfft_malloc(staging)");
+        m_owned = true;
+        if (in)
+        {
+            const crdfft_status st = crdfft_memcpy_h2d(m_dev, p, bytes);
+            if (st != CRDFFT_OK)
+            {
+                (void)crdfft_free(m_dev);
+                throw FftError(st, "crdfft_memcpy_h2d(staging)");
+            }
+        }
+    }
+    ~Staged()
+    {
+        if (m_owned)
+        {
+            (void)crdfft_free(m_dev);
+        }
+    }
+    Staged(const Staged&) = delete;
+    Staged& operator=(const Staged&) = delete;
+    [[nodiscard]] void* device() const noexcept { return m_dev; }
+    [[nodiscard]] bool staged() const noexcept { return m_owned; }
+    void finish()
+    {
+        if (!m_owned)
+        {
+            return;
+        }
+        check(crdfft_stream_synchronize(m_stream), "crdfft_stream_synchronize");
+        if (m_out)
+        {
+            check(crdfft_memcpy_d2h(m_user, m_dev, m_bytes), "crdfft_memcpy_d2h(staging)");
+        }
+    }
+
+private:
+    void* m_user;
+    void* m_dev = nullptr;
+    std::size_t m_bytes;
+    bool m_out;
+    void* m_stream;
+    bool m_owned = false;
+};
+
+// How a reference FftAlgorithm hint maps onto a device plan of n (the hint is a CPU engine; every device plan kind
+// computes the same transform):
+//   Stockham, StandaloneHier, Ip4, DeepSplit3, DeepSplit4, InPlace3, TiledFourStep -> the planner's plan (one
+//     LDS-resident kernel up to n = 16384, the two-pass four-step above);
+//   FourStep -> the four-step (CRDFFT_ALGO_FOUR_STEP) where n > 16384, else the planner's;
+//   SixStep  -> the four-step with the square split n1 = 2^floor(log2(n)/2) (the reference's six-step changes only
+//     the split, fft.hpp:233-236) where n > 16384 and that split has kernels, else the planner's.
+inline crdfft_tuning algorithm_tuning(std::uint64_t n, FftAlgorithm algorithm)
+{
+    crdfft_tuning t{};
+    if (n <= 16384 || (algorithm != FftAlgorithm::FourStep && algorithm != FftAlgorithm::SixStep))
+    {
+        return t;
+    }
+    t.algorithm = CRDFFT_ALGO_FOUR_STEP;
+    if (algorithm == FftAlgorithm::SixStep)
+    {
+        unsigned lg = 0;
+        while ((std::uint64_t{1} << lg) < n)
+        {
+            ++lg;
+        }
+        crdfft_tuning sq = t;
+        sq.n1 = std::uint64_t{1} << (lg / 2);
+        crdfft_plan_info info{};
+        if (crdfft_plan_describe_tuned(n, 1, CRDFFT_C2C, CRDFFT_FP32, &sq, &info) == CRDFFT_OK)
+        {
+            return sq;
+        }
+    }
+    return t;
+}
+} // namespace detail
+
 template <typename T> class FftPlan
 {
     static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
 
 public:
+    // FftPlan(IAllocator*, n) (fft.hpp:224-227): one transform of n, the planner's plan; any allocator type.
+    template <typename Alloc> FftPlan(Alloc* /*alloc*/, std::size_t n) : FftPlan(n, 1, nullptr) {}
+    // FftPlan(IAllocator*, n, FftAlgorithm) (fft.hpp:232): the hint mapped by detail::algorithm_tuning.
+    template <typename Alloc>
+    FftPlan(Alloc* /*alloc*/, std::size_t n, FftAlgorithm forward) : m_n(n), m_batch(1), m_stream(nullptr)
+    {
+        const crdfft_tuning t = detail::algorithm_tuning(n, forward);
+        check(crdfft_plan_create_tuned(&m_plan, n, 1, 0, 0, CRDFFT_C2C, CRDFFT_FP32, CRDFFT_FLAG_NONE, &t),
+              "crdfft_plan_create_tuned");
+    }
+
     // A plan of `batch` contiguous transforms of n (fft.hpp:224-227); `stream` is a hipStream_t or null.
     explicit FftPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr) : m_n(n), m_batch(batch), m_stream(stream)
     {
@@ -164,6 +326,30 @@ public:
         check(crdfft_execute_reference(m_plan, in.data(), out.data(), static_cast<int32_t>(dir), m_stream),
               "crdfft_execute_reference");
     }
+    // The reference's call shapes on any span (fft.hpp:409-413, :462-470, :417-460): in place, host or device memory.
+    template <SpanOf<8> S> void execute(S&& data, FftDirection dir) const
+    {
+        require(data.size() == m_n * m_batch, "FftPlan::execute: size mismatch");
+        detail::Staged d(data.data(), data.size() * 8, true, true, m_stream);
+        check(crdfft_execute(m_plan, d.device(), d.device(), static_cast<int32_t>(dir), m_stream), "crdfft_execute");
+        d.finish();
+    }
+    template <SpanOf<8> S> void execute_batched(S&& data, std::size_t batch, FftDirection dir) const
+    {
+        require(data.size() == m_n * batch, "FftPlan::execute_batched: size mismatch");
+        detail::Staged d(data.data(), data.size() * 8, true, true, m_stream);
+        check(crdfft_execute_batched(m_plan, d.device(), batch, static_cast<int32_t>(dir), m_stream),
+              "crdfft_execute_batched");
+        d.finish();
+    }
+    template <SpanOf<8> S> void execute_reference(S&& data, FftDirection dir) const
+    {
+        require(data.size() == m_n * m_batch, "FftPlan::execute_reference: size mismatch");
+        detail::Staged d(data.data(), data.size() * 8, true, true, m_stream);
+        check(crdfft_execute_reference(m_plan, d.device(), d.device(), static_cast<int32_t>(dir), m_stream),
+              "crdfft_execute_reference");
+        d.finish();
+    }
     [[nodiscard]] crdfft_plan_info state() const
     {
         crdfft_plan_info info{};
@@ -192,6 +378,8 @@ template <typename T> class RealFftPlan
     static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
 
 public:
+    // RealFftPlan(IAllocator*, n) (real_fft.hpp:21-36): one transform.
+    template <typename Alloc> RealFftPlan(Alloc* /*alloc*/, std::size_t n) : RealFftPlan(n, 1, nullptr) {}
     explicit RealFftPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr) : m_n(n), m_batch(batch), m_stream(stream)
     {
         check(crdfft_plan_create(&m_r2c, n, batch, 0, 0, CRDFFT_R2C, CRDFFT_FP32, CRDFFT_FLAG_NONE),
@@ -230,6 +418,31 @@ public:
             throw FftError(CRDFFT_INVALID_ARGUMENT, "RealFftPlan::irfft: size mismatch");
         }
         check(crdfft_execute(m_c2r, in.data(), real_out.data(), CRDFFT_INVERSE, m_stream), "crdfft_execute(C2R)");
+    }
+    // The reference's call shapes on any span (real_fft.hpp:41, :67), host or device memory.
+    template <SpanOf<4> In, SpanOf<8> Out> void rfft(In&& real_in, Out&& out) const
+    {
+        if (real_in.size() != m_n * m_batch || out.size() != (m_n / 2 + 1) * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "RealFftPlan::rfft: size mismatch");
+        }
+        detail::Staged x(real_in.data(), real_in.size() * 4, true, false, m_stream);
+        detail::Staged y(out.data(), out.size() * 8, false, true, m_stream);
+        check(crdfft_execute(m_r2c, x.device(), y.device(), CRDFFT_FORWARD, m_stream), "crdfft_execute(R2C)");
+        y.finish();
+        x.finish();
+    }
+    template <SpanOf<8> In, SpanOf<4> Out> void irfft(In&& in, Out&& real_out) const
+    {
+        if (in.size() != (m_n / 2 + 1) * m_batch || real_out.size() != m_n * m_batch)
+        {
+            throw FftError(CRDFFT_INVALID_ARGUMENT, "RealFftPlan::irfft: size mismatch");
+        }
+        detail::Staged x(in.data(), in.size() * 8, true, false, m_stream);
+        detail::Staged y(real_out.data(), real_out.size() * 4, false, true, m_stream);
+        check(crdfft_execute(m_c2r, x.device(), y.device(), CRDFFT_INVERSE, m_stream), "crdfft_execute(C2R)");
+        y.finish();
+        x.finish();
     }
 
 private:
@@ -270,6 +483,8 @@ template <typename T> class BluesteinPlan
     static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
 
 public:
+    // BluesteinPlan(IAllocator*, n) (bluestein.hpp:32): one transform.
+    template <typename Alloc> BluesteinPlan(Alloc* /*alloc*/, std::size_t n) : BluesteinPlan(n, 1, nullptr) {}
     explicit BluesteinPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr)
         : m_n(n), m_batch(batch), m_stream(stream)
     {
@@ -277,14 +492,17 @@ public:
               "crdfft_bluestein_plan_create");
     }
     [[nodiscard]] std::size_t size() const noexcept { return m_n; }
-    void execute(DeviceSpan<Complex<T>> data, FftDirection dir) const
+    // bluestein.hpp:87, in place on any span, host or device memory
+    template <SpanOf<8> S> void execute(S&& data, FftDirection dir) const
     {
         if (data.size() != m_n * m_batch)
         {
             throw FftError(CRDFFT_INVALID_ARGUMENT, "BluesteinPlan::execute: size mismatch");
         }
-        check(crdfft_execute(m_plan.get(), data.data(), data.data(), static_cast<int32_t>(dir), m_stream),
+        detail::Staged d(data.data(), data.size() * 8, true, true, m_stream);
+        check(crdfft_execute(m_plan.get(), d.device(), d.device(), static_cast<int32_t>(dir), m_stream),
               "crdfft_execute(Bluestein)");
+        d.finish();
     }
 
 private:
@@ -315,13 +533,29 @@ public:
         }
         check(crdfft_nd_plan_create(m_plan.out(), d, ndim, batch, 0), "crdfft_nd_plan_create");
     }
-    void execute(DeviceSpan<Complex<T>> data, FftDirection dir) const
+    // NdFftPlan(IAllocator*, ConstSpan<usize> dims) (nd_fft.hpp:31): one volume; the reference's job_plans count
+    // (host threads) has no device meaning and is accepted and ignored.
+    template <typename Alloc, typename Dims>
+        requires requires(const Dims& d) {
+            { d.data() };
+            { d.size() } -> std::convertible_to<std::size_t>;
+        }
+    NdFftPlan(Alloc* /*alloc*/, const Dims& dims, std::uint32_t /*job_plans*/ = 1)
+        : NdFftPlan(reinterpret_cast<const std::size_t*>(dims.data()), static_cast<std::uint32_t>(dims.size()), 1,
+                    nullptr)
+    {
+        static_assert(sizeof(*dims.data()) == sizeof(std::size_t), "dims are usize");
+    }
+    // nd_fft.hpp:105, in place on any span, host or device memory
+    template <SpanOf<8> S> void execute(S&& data, FftDirection dir) const
     {
         if (data.size() != m_total)
         {
             throw FftError(CRDFFT_INVALID_ARGUMENT, "NdFftPlan::execute: size mismatch");
         }
-        check(crdfft_nd_execute(m_plan.get(), data.data(), static_cast<int32_t>(dir), m_stream), "crdfft_nd_execute");
+        detail::Staged d(data.data(), data.size() * 8, true, true, m_stream);
+        check(crdfft_nd_execute(m_plan.get(), d.device(), static_cast<int32_t>(dir), m_stream), "crdfft_nd_execute");
+        d.finish();
     }
 
 private:
@@ -336,25 +570,34 @@ template <typename T> class DctPlan
     static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
 
 public:
+    // DctPlan(IAllocator*, n) (dct.hpp:38): one row.
+    template <typename Alloc> DctPlan(Alloc* /*alloc*/, std::size_t n) : DctPlan(n, 1, nullptr) {}
     explicit DctPlan(std::size_t n, std::size_t batch = 1, void* stream = nullptr)
         : m_n(n), m_batch(batch), m_stream(stream)
     {
         check(crdfft_dct_plan_create(m_plan.out(), n, batch, 0, 0, CRDFFT_FLAG_NONE), "crdfft_dct_plan_create");
     }
     [[nodiscard]] std::size_t size() const noexcept { return m_n; }
-    void dct2(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DCT2, x, y); }
-    void dct3(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DCT3, x, y); }
-    void dst2(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DST2, x, y); }
-    void dst3(DeviceSpan<const T> x, DeviceSpan<T> y) const { run(CRDFFT_DST3, x, y); }
+    // dct.hpp:62, :99, :120, :164 on any spans (ConstSpan in, Span out), host or device memory
+    template <SpanOf<4> X, SpanOf<4> Y> void dct2(X&& x, Y&& y) const { run(CRDFFT_DCT2, x, y); }
+    template <SpanOf<4> X, SpanOf<4> Y> void dct3(X&& x, Y&& y) const { run(CRDFFT_DCT3, x, y); }
+    template <SpanOf<4> X, SpanOf<4> Y> void dst2(X&& x, Y&& y) const { run(CRDFFT_DST2, x, y); }
+    template <SpanOf<4> X, SpanOf<4> Y> void dst3(X&& x, Y&& y) const { run(CRDFFT_DST3, x, y); }
 
 private:
-    void run(crdfft_r2r_kind kind, DeviceSpan<const T> x, DeviceSpan<T> y) const
+    template <typename X, typename Y> void run(crdfft_r2r_kind kind, const X& x, const Y& y) const
     {
         if (x.size() != m_n * m_batch || y.size() != m_n * m_batch)
         {
             throw FftError(CRDFFT_INVALID_ARGUMENT, "DctPlan: size mismatch");
         }
-        check(crdfft_dct_execute(m_plan.get(), kind, x.data(), y.data(), m_stream), "crdfft_dct_execute");
+        detail::Staged xs(x.data(), x.size() * 4, true, false, m_stream);
+        detail::Staged ys(y.data(), y.size() * 4, false, true, m_stream);
+        check(crdfft_dct_execute(m_plan.get(), kind, static_cast<const float*>(xs.device()),
+                                 static_cast<float*>(ys.device()), m_stream),
+              "crdfft_dct_execute");
+        ys.finish();
+        xs.finish();
     }
     std::size_t m_n;
     std::size_t m_batch;
@@ -425,6 +668,23 @@ template <typename T> void ifft_normalized(DeviceSpan<Complex<T>> data)
     (void)crdfft_plan_destroy(plan);
     check(st, "crdfft_execute");
     check(sy, "crdfft_device_synchronize");
+}
+
+// The reference's one-shot calls (fft.hpp:1005-1029), with its allocator argument and any span, host or device
+// memory; synchronous like the reference.
+template <typename T, typename Alloc, SpanOf<8> S>
+void fft(Alloc* alloc, S&& data, FftDirection dir = FftDirection::Forward)
+{
+    const FftPlan<T> plan(alloc, data.size());
+    plan.execute(data, dir);
+    check(crdfft_device_synchronize(), "crdfft_device_synchronize");
+}
+template <typename T, typename Alloc, SpanOf<8> S> void ifft_normalized(Alloc* /*alloc*/, S&& data)
+{
+    static_assert(std::is_same_v<T, float>, "the MI355X FFT path is FP32");
+    detail::Staged d(data.data(), data.size() * 8, true, true, nullptr);
+    ifft_normalized<T>(DeviceSpan<Complex<T>>(static_cast<Complex<T>*>(d.device()), data.size()));
+    d.finish();
 }
 
 } // namespace crd_amd::hesap::fft

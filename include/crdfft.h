@@ -120,7 +120,23 @@ typedef struct crdfft_plan_info
     uint64_t table_bytes;
     uint32_t hints;       /* cache hints: bit 0 non-temporal input loads, bit 1 non-temporal output stores;
                              four-step bit 5 (CRDFFT_HINT_TWO_PIPELINES): alternate chunks run on a second stream */
+    int32_t check;        /* crdfft_check_status of the creation-time self-check */
 } crdfft_plan_info;
+
+/* What the creation-time known-answer check found (KernelCheckStatus, engine/foundation/core/include/crd/core/
+ * kernel_dispatch.hpp:83-99). The plan's selected kernels are checked first; when they fail, the plan drops to its
+ * floor, the radix-2 reference kernels of crdfft_execute_reference (bit reversal + butterfly stages over device
+ * memory, with the plan's 1/n and the separate real-transform passes around them), is checked again and runs
+ * there: DEMOTED. When the floor fails too, creation returns CRDFFT_SELF_CHECK_FAILED (the reference runs its
+ * portable table then; a device whose simplest kernels misexecute has nothing safer to offer). Fault seam:
+ * CRDFFT_FAULT_SELF_CHECK=1 in the environment fails the selected kernels' check (the plan demotes), =2 fails
+ * every check. Element-major batches (crdfft_execute_batched) keep their own kernels. */
+typedef enum crdfft_check_status
+{
+    CRDFFT_CHECK_NOT_RUN = 0, /* CRDFFT_FLAG_NO_SELF_CHECK, or a plan kind the check does not cover */
+    CRDFFT_CHECK_PASSED = 1,  /* the selected kernels passed and run */
+    CRDFFT_CHECK_DEMOTED = 2  /* the selected kernels failed; the floor passed and runs */
+} crdfft_check_status;
 
 /* Tuning overrides for crdfft_plan_create_tuned; a zero field keeps the planner's choice. */
 #define CRDFFT_HINTS_NONE 0x100u /* crdfft_tuning.hints: no cache hints (plain loads and stores) */
@@ -175,7 +191,9 @@ CRDFFT_API crdfft_status crdfft_execute(crdfft_plan plan, const void* in, void* 
  * differ from it (the reference passes it per call). C2C plans only. */
 CRDFFT_API crdfft_status crdfft_execute_batched(crdfft_plan plan, void* data, uint64_t batch, int32_t direction,
                                                 void* stream);
-/* The radix-2 reference transform of the plan's batch, out of place (in != out). C2C plans only. */
+/* The radix-2 reference transform of the plan's batch (fft.hpp:417-460): bit reversal, then log2(n) butterfly
+ * stages. in == out runs it in place (the bit reversal by pair swaps, as the reference does on the caller's span);
+ * otherwise out of place. C2C plans only. */
 CRDFFT_API crdfft_status crdfft_execute_reference(crdfft_plan plan, const void* in, void* out, int32_t direction,
                                                   void* stream);
 /* Known-answer check of the plan's kernels (synchronous; allocates its own small buffers). */

@@ -31,6 +31,9 @@ CRDFFT_API crdfft_status crdfft_device_props_get(int32_t device, crdfft_device_p
 
 CRDFFT_API crdfft_status crdfft_malloc(void** ptr, uint64_t bytes);
 CRDFFT_API crdfft_status crdfft_free(void* ptr);
+/* *is_device = 1 when ptr points into device (or managed) memory that kernels can read, 0 for host memory (pageable
+ * or pinned) and for NULL. The C++ interface uses it to run reference call sites that pass host spans. */
+CRDFFT_API crdfft_status crdfft_pointer_is_device(const void* ptr, int32_t* is_device);
 /* Synchronous copies (host <-> device) and an asynchronous device copy on `stream`. */
 CRDFFT_API crdfft_status crdfft_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 CRDFFT_API crdfft_status crdfft_memcpy_d2h(void* dst, const void* src, uint64_t bytes);

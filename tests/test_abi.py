@@ -167,14 +167,70 @@ def test_null_handles_are_rejected_without_a_device():
     assert L.crdfft_graph_destroy(None) == N.INVALID_ARGUMENT
 
 
-def test_cpp_interface_compiles():
+@pytest.mark.parametrize("name", ["wrapper_use.cpp", "reference_call_sites.cpp"])
+def test_cpp_interface_compiles(name):
+    """Both host programs compile against the C++ interface: the device-native forms (wrapper_use.cpp) and call
+    sites written exactly as the reference's tests write them, over host spans (reference_call_sites.cpp)."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
-    src = os.path.join(ROOT, "tests", "cpp", "wrapper_use.cpp")
+    src = os.path.join(ROOT, "tests", "cpp", name)
     r = subprocess.run([gxx, "-std=c++20", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                         "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_algorithm_hints_map_onto_device_plans(tmp_path):
+    """FftPlan(alloc, n, FftAlgorithm) (fft.hpp:232): every one of the nine hints yields a plan the planner accepts,
+    FourStep and SixStep select the four-step above n = 16384, and SixStep takes the square split there (the
+    reference's six-step changes only the split, fft.hpp:233-236). Host-only: the mapping goes through
+    crdfft_plan_describe_tuned, which touches no device."""
+    from crd_amd import _native as N
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    code = r"""
+#include <crd_amd/fft.hpp>
+#include <cstdio>
+using namespace crd_amd::hesap::fft;
+int main()
+{
+    for (int a = 0; a < 9; ++a)
+    {
+        for (unsigned lg : {4u, 12u, 16u, 17u, 20u})
+        {
+            const crdfft_tuning t = detail::algorithm_tuning(1ull << lg, static_cast<FftAlgorithm>(a));
+            crdfft_plan_info info{};
+            const int st = crdfft_plan_describe_tuned(1ull << lg, 1, CRDFFT_C2C, CRDFFT_FP32, &t, &info);
+            std::printf("%s %u %d %d %llu\n", fft_algorithm_name(static_cast<FftAlgorithm>(a)), lg, st,
+                        info.algorithm, static_cast<unsigned long long>(info.n1));
+        }
+    }
+    return 0;
+}
+"""
+    exe = str(tmp_path / "algo_hints")
+    libdir = os.path.join(ROOT, "crd_amd")
+    r = subprocess.run([gxx, "-std=c++20", "-x", "c++", "-", "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                        "-lcrdfft", f"-Wl,-rpath,{libdir}", "-o", exe], input=code, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    rows = [ln.split() for ln in out.stdout.strip().splitlines()]
+    assert len(rows) == 45
+    for name, lg, st, algo, n1 in rows:
+        lg, st, algo, n1 = int(lg), int(st), int(algo), int(n1)
+        assert st == N.OK, (name, lg)
+        if lg <= 14:
+            assert algo == 1, (name, lg)  # one LDS-resident kernel
+        else:
+            assert algo == 2, (name, lg)  # the four-step
+        if name == "SixStep" and lg > 14:
+            assert n1 == 1 << (lg // 2), (name, lg, n1)
+    names = sorted({r[0] for r in rows})
+    assert names == sorted(["Stockham", "StandaloneHier", "Ip4", "DeepSplit3", "DeepSplit4", "FourStep", "SixStep",
+                            "TiledFourStep", "InPlace3"])
 
 
 def test_c_header_compiles_as_c():

@@ -66,17 +66,21 @@ def test_one_adam_step_matches_oracle_gradient_step(gpu, po):
 
 
 def test_ranking_gains_over_no_model(gpu):
-    """Trained on timed candidates of seeded shapes, the model's top-2 picks (timing 2 candidates per case) must
-    land closer to the exhaustive best than 2 random candidates do, on held-out shapes."""
+    """What the trained network gains over the method without it: trained on the timed candidates of seeded shapes,
+    the plan the model ranks first (one candidate timed per case) lands closer to the exhaustive best than the
+    planner's default plan (candidate 0, what runs without the model) on held-out shapes, and so does the best of
+    its top 2; the top 2 also beat two random candidates, while timing under 75 % of the candidates."""
     from crd_amd import costmodel
 
-    train = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(60, seed=11, lo=8, hi=20)]
-    held = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(40, seed=12, lo=8, hi=20)]
+    train = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(240, seed=11, lo=8, hi=20)]
+    held = [costmodel.time_case(c, reps=3) for c in costmodel.draw_cases(80, seed=12, lo=8, hi=20)]
     recs = [r for rs in train for r in rs]
     x, y = costmodel.features(recs), costmodel.target(recs)
     model = costmodel.Mlp(seed=1)
     model.train(x, (y - y.mean()) / (y.std() + 1e-6), epochs=300, batch=128, lr=2e-3, seed=7)
-    rep = costmodel.evaluate(model, held, k_values=(2,))
+    rep = costmodel.evaluate(model, held, k_values=(1, 2))
     print(rep)
+    assert rep["top1"]["mean_regret"] < rep["default_mean_regret"]
+    assert rep["top2"]["mean_regret"] < rep["default_mean_regret"]
     assert rep["top2"]["mean_regret"] <= rep["top2"]["random_mean_regret"]
     assert rep["top2"]["timed_fraction"] < 0.75

@@ -118,8 +118,13 @@ def test_specials_propagate(gpu, po):
         assert np.all(yz == 0)
 
 
-@pytest.mark.parametrize("n,batch", [(2, 3), (16, 19), (64, 17), (256, 16), (1024, 7), (4096, 5)])
+@pytest.mark.parametrize("n,batch", [(2, 3), (16, 19), (64, 17), (256, 16), (1024, 7), (4096, 5), (8192, 5),
+                                     (8192, 17), (1 << 14, 100), (1 << 15, 3), (1 << 17, 2), (1 << 20, 3),
+                                     (1 << 22, 1), (1 << 24, 1)])
 def test_element_major_batched(gpu, po, n, batch):
+    """execute_batched, element-major data[i*batch + t] (fft.hpp:462-470): one strided-column kernel up to n = 4096,
+    the two-pass strided four-step above (the reference's engine takes every power of two, fft_kernels.cpp:593-784),
+    against FP64 and the oracle's FP32 Plan32::execute_batched, both directions, odd batches included."""
     x = po.lcg_complex(n * batch, n ^ batch)
     plan = gpu.FftPlan(n, 1)
     buf = gpu.DeviceBuffer.from_array(x)
@@ -149,6 +154,12 @@ def test_reference_radix2_on_device(gpu, po, lg):
         assert po.rel_l2(y, po.fft64_of_f32(x, inverse=inverse)) <= bound(n)
         r = po.Plan32(n).execute_reference(x[0], inverse)
         assert po.rel_l2(y[0], r) <= 2 * bound(n)
+        # in place (fft.hpp:417-460 works on the caller's span): the pair-swap bit reversal, then the same stages,
+        # so the bits equal the out-of-place form's
+        inplace = gpu.DeviceBuffer.from_array(x)
+        plan.execute_reference(inplace, None, gpu.INVERSE if inverse else gpu.FORWARD)
+        gpu.synchronize()
+        assert np.array_equal(inplace.download(x.shape, np.complex64).view(np.uint32), y.view(np.uint32))
 
 
 @pytest.mark.parametrize("passes", [False, True])
@@ -293,7 +304,7 @@ def test_error_paths(gpu):
     with pytest.raises(N.CrdfftError):
         c2c.execute(buf, buf, 5)
     with pytest.raises(N.CrdfftError) as e:
-        c2c.execute_reference(buf, buf)
+        c2c.execute_reference(buf, buf, 7)  # no such direction
     assert e.value.status == N.INVALID_ARGUMENT
 
 
@@ -430,29 +441,100 @@ def test_allocation_failures_are_reported(gpu):
 
 
 def test_self_check_fault_seam(gpu):
-    """CRDFFT_FAULT_SELF_CHECK=1 flips an output bit on the host: plan creation must fail with SELF_CHECK_FAILED
-    (the reference forces its demotion path the same way, fault_seam.hpp:139)."""
+    """The demotion path (select_checked_kernel_level, kernel_dispatch.hpp:105-145), forced by the host fault seam
+    as the reference forces its own (fault_seam.hpp:139): CRDFFT_FAULT_SELF_CHECK=1 fails the selected kernels'
+    check, so the plan is created on its floor (the radix-2 reference kernels) and reports DEMOTED; =2 fails the
+    floor's check too, and creation refuses the plan."""
     from crd_amd import _native as N
 
     for n, t in ((4096, "c2c"), (1 << 18, "c2c"), (1024, "r2c"), (1024, "c2r")):
-        os.environ["CRDFFT_FAULT_SELF_CHECK"] = "1"
+        os.environ["CRDFFT_FAULT_SELF_CHECK"] = "2"
         try:
             with pytest.raises(N.CrdfftError) as e:
                 gpu.FftPlan(n, 2, t)
             assert e.value.status == N.SELF_CHECK_FAILED
         finally:
             del os.environ["CRDFFT_FAULT_SELF_CHECK"]
+        os.environ["CRDFFT_FAULT_SELF_CHECK"] = "1"
+        try:
+            demoted = gpu.FftPlan(n, 2, t)
+        finally:
+            del os.environ["CRDFFT_FAULT_SELF_CHECK"]
+        assert demoted.info()["check"] == N.CHECK_DEMOTED
+        assert demoted.self_check() == N.OK  # the floor, checked again without the seam
         p = gpu.FftPlan(n, 2, t)
+        assert p.info()["check"] == N.CHECK_PASSED
         assert p.self_check() == N.OK
 
 
-def test_cpp_interface_runs(gpu, tmp_path):
-    """The C++20 interface (include/crd_amd/fft.hpp) drives the library from a host program."""
+@pytest.mark.parametrize("n,t,normalize", [(64, "c2c", False), (4096, "c2c", True), (1 << 17, "c2c", False),
+                                           (1 << 18, "c2c", True), (1024, "r2c", False), (1 << 16, "r2c", False),
+                                           (1024, "c2r", False), (1 << 16, "c2r", False)])
+def test_demoted_plan_matches_fp64(gpu, po, n, t, normalize):
+    """A plan demoted to its floor computes the same transforms, against FP64: C2C both directions (in place, out
+    of place, the normalized inverse with padded rows whose padding must stay untouched), R2C and C2R through the
+    separate recombine passes around the floor's complex transform."""
+    from crd_amd import _native as N
+
+    batch = 3
+    os.environ["CRDFFT_FAULT_SELF_CHECK"] = "1"
+    try:
+        if t == "c2c":
+            dist = n + 8
+            flags = N.FLAG_NORMALIZE_INVERSE if normalize else 0
+            plan = gpu.FftPlan(n, batch, t, idist=dist, odist=dist, flags=flags)
+        else:
+            plan = gpu.FftPlan(n, batch, t)
+    finally:
+        del os.environ["CRDFFT_FAULT_SELF_CHECK"]
+    assert plan.info()["check"] == N.CHECK_DEMOTED
+    if t == "c2c":
+        x = np.zeros((batch, dist), np.complex64)
+        x[:, :n] = po.lcg_complex(n * batch, n).reshape(batch, n)
+        x[:, n:] = np.complex64(7 + 7j)  # guard words
+        for inverse in (False, True):
+            src = gpu.DeviceBuffer.from_array(x)
+            dst = gpu.DeviceBuffer.from_array(x)
+            d = gpu.INVERSE if inverse else gpu.FORWARD
+            plan.execute(src, dst, d)
+            plan.execute(src, None, d)  # in place
+            gpu.synchronize()
+            ref = po.fft64_of_f32(x[:, :n], inverse=inverse)
+            if inverse and normalize:
+                ref = ref / n
+            for buf in (dst, src):
+                y = buf.download(x.shape, np.complex64)
+                assert po.rel_l2(y[:, :n], ref) <= bound(n)
+                assert np.array_equal(y[:, n:], x[:, n:])
+    elif t == "r2c":
+        x = po.normal_real(n * batch, n).reshape(batch, n)
+        out = gpu.DeviceBuffer(8 * (n // 2 + 1) * batch)
+        plan.execute(gpu.DeviceBuffer.from_array(x), out, gpu.FORWARD)
+        gpu.synchronize()
+        y = out.download((batch, n // 2 + 1), np.complex64)
+        assert po.rel_l2(y, np.fft.rfft(x.astype(np.float64), axis=1)) <= bound(n)
+    else:
+        spec = np.fft.rfft(po.normal_real(n * batch, n).reshape(batch, n).astype(np.float64), axis=1)
+        spec = spec.astype(np.complex64)
+        out = gpu.DeviceBuffer(4 * n * batch)
+        plan.execute(gpu.DeviceBuffer.from_array(spec), out, gpu.INVERSE)
+        gpu.synchronize()
+        y = out.download((batch, n), np.float32)
+        assert po.rel_l2(y, np.fft.irfft(spec.astype(np.complex128), n=n, axis=1)) <= bound(n)
+
+
+@pytest.mark.parametrize("name", ["wrapper_use", "reference_call_sites"])
+def test_cpp_interface_runs(gpu, tmp_path, name):
+    """The C++20 interface (include/crd_amd/fft.hpp) drives the library from host programs: the device-native forms,
+    and call sites written exactly as the reference's tests write them over host spans (FftPlan(alloc, n),
+    FftAlgorithm hints, fft / ifft_normalized with the allocator, in-place execute_reference, element-major
+    execute_batched up to 8192, RealFftPlan / BluesteinPlan / NdFftPlan / DctPlan), checked against an FP64 DFT at
+    the reference's f32 gate."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
-    exe = str(tmp_path / "wrapper_use")
-    src = os.path.join(ROOT, "tests", "cpp", "wrapper_use.cpp")
+    exe = str(tmp_path / name)
+    src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
     libdir = os.path.join(ROOT, "crd_amd")
     subprocess.run([gxx, "-std=c++20", "-O1", "-I", os.path.join(ROOT, "include"), src, "-L", libdir, "-lcrdfft",
                     f"-Wl,-rpath,{libdir}", "-o", exe], check=True)

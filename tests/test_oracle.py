@@ -160,6 +160,41 @@ def test_real32_against_fp64(po, n):
     assert np.max(np.abs(back - x)) <= 2e-6 * np.log2(n) * np.max(np.abs(x))
 
 
+def _mlp_params(inputs, hidden):
+    """Zero parameters and views of each tensor of the flat layout: W1[hidden][inputs], b1[hidden],
+    W2[hidden][hidden] (row = output unit), b2[hidden], w3[hidden], b3."""
+    sizes = [hidden * inputs, hidden, hidden * hidden, hidden, hidden, 1]
+    p = np.zeros(sum(sizes))
+    parts = np.split(p, np.cumsum(sizes)[:-1])
+    return p, (parts[0].reshape(hidden, inputs), parts[1], parts[2].reshape(hidden, hidden), parts[3], parts[4],
+               parts[5])
+
+
+def test_mlp_forward_closed_forms(po):
+    """The oracle's forward pass on parameters whose output is known in closed form: all weights zero gives b3 for
+    every row; one active path x_i -> unit j -> unit k -> output gives e * relu(d * relu(a x_i + c) + q) + f, and the
+    same path with a negative pre-activation gives f (the ReLU clips). Inputs differ from hidden units in count and
+    the path's indices are distinct, so a transposed W1 or W2, a dropped bias or a missing ReLU fails."""
+    inputs, hidden = 7, 5
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-2, 2, (13, inputs))
+    p, (w1, b1, w2, b2, w3, b3) = _mlp_params(inputs, hidden)
+    b3[0] = 0.625
+    assert np.all(po.mlp_forward(p, inputs, hidden, x) == 0.625)
+    i, j, k = 6, 1, 3
+    a, c, d, q, e, f = 1.5, 0.25, -2.0, 0.5, 0.75, -0.125
+    w1[j, i], b1[j], w2[k, j], b2[k], w3[k], b3[0] = a, c, d, q, e, f
+    relu = lambda v: np.maximum(v, 0.0)
+    want = e * relu(d * relu(a * x[:, i] + c) + q) + f
+    got = po.mlp_forward(p, inputs, hidden, x)
+    assert np.max(np.abs(got - want)) <= 1e-14
+    assert np.any(want != f) and np.any(want == f)  # both sides of the second ReLU occur in the sample
+    # inputs other than x_i have no path to the output
+    x2 = x.copy()
+    x2[:, [m for m in range(inputs) if m != i]] = rng.uniform(-2, 2, (13, inputs - 1))
+    assert np.array_equal(po.mlp_forward(p, inputs, hidden, x2), got)
+
+
 def test_mlp_gradient_matches_finite_differences(po):
     """The oracle's backpropagation against central differences of its own forward (FP64)."""
     inputs, hidden, n = 7, 5, 11
