@@ -1,4 +1,6 @@
-// Host view of the split-precision MFMA passes (mfma_kernels.hip): the four-step of n = 256 x 256.
+// Host view of the split-precision MFMA passes (mx_kernels.hip): radix-16 (256-point) and radix-32 (1024-point)
+// sequences as DFT-matrix products, one whole transform of 256 or 1024 points, or the two passes of a four-step
+// n = n1 * n2 with n1, n2 in {256, 1024}.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,20 +18,25 @@ struct SplitArgs
 {
     const cfloat* in;
     cfloat* out;
-    const float* tab;     // operand splits of the DFT matrices (build_split_tables), this direction's
-    const cfloat* tw256;  // W_256^e, e < 256
-    const cfloat* thi;    // four-step twiddle W_65536^{j << hshift}
-    const cfloat* tlo;    // W_65536^{j}, j < 2^hshift
+    const float* tab;    // operand splits of the radix's DFT matrices (build_split_tables), this direction's
+    const cfloat* tws;   // W_S^e, e < S: the stage twiddles of the sequence length S
+    const cfloat* thi;   // pass 0: four-step twiddle W_n^{j << hshift}
+    const cfloat* tlo;   // pass 0: W_n^{j}, j < 2^hshift
     uint32_t hshift;
-    uint64_t count;       // 65536-point transforms
+    uint64_t count;      // transforms (pass 2) or four-step matrices (passes 0, 1)
     uint64_t idist;
     uint64_t odist;
+    uint64_t n_seq;      // pass 0: columns (n2); pass 1: rows (n1); pass 2: 1
+    float scale;         // pass 1 and 2: applied to every output
 };
 
-uint64_t split_table_floats();
-hipError_t build_split_tables(float* tab, int mode, bool inverse, hipStream_t s);
+// The radix of a sequence length (256 -> 16, 1024 -> 32), false for other lengths.
+bool split_radix_of(uint64_t len, int* radix);
+uint64_t split_table_floats(int radix);
+hipError_t build_split_tables(float* tab, int radix, int mode, bool inverse, hipStream_t s);
 // pass 0: column pass with the four-step twiddle (in: the transforms, out: the intermediate, same layout);
-// pass 1: row pass with the transposed store (in: the intermediate, out: the transforms)
-hipError_t launch_split_pass(int pass, int mode, bool inverse, const SplitArgs& a, hipStream_t s);
+// pass 1: row pass with the transposed store (in: the intermediate, out: the transforms);
+// pass 2: whole transforms of one sequence each
+hipError_t launch_split_pass(int pass, int radix, int mode, bool inverse, const SplitArgs& a, hipStream_t s);
 
 } // namespace crdfft

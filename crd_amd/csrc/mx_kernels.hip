@@ -1,0 +1,655 @@
+// Split-precision MFMA passes of the four-step FFT: radix-R stages as complex R x R DFT-matrix products on gfx950's
+// matrix cores, accumulated in FP32, with the sequence kept in one wave's registers from load to store.
+//
+// A sequence of S = R*R points (R = 16: S = 256 on v_mfma_f32_16x16x32_{f16,bf16}; R = 32: S = 1024 on
+// v_mfma_f32_32x32x16_{f16,bf16}) is the R x R matrix X[a][b], point n = R*a + b. Stage A forms
+// Y[b][k1] = sum_a X[a][b] W_R^{a k1} for every b and multiplies it by W_S^{b k1}; stage B forms
+// Z[k1][k2] = sum_b Y[b][k1] W_R^{b k2}, and X_out[k1 + R*k2] = Z[k1][k2]. Each stage is one matrix product
+// D = A * B with A = the data (rows: the index kept, K: the contracted index, real parts then imaginary parts
+// stacked along K) and B = [Fr; -Fi] for the real part of the result, [Fi; Fr] for the imaginary part.
+//
+// Layouts (the MFMA operand and result layouts of gfx950): the result D[i][j] of a product sits in lane l, register
+// r at j = l % R, i = row(l, r) (R = 16: 4*(l/16) + r; R = 32: 8*(r/4) + 4*(l/32) + r%4). A sequence is loaded into
+// that same layout (i = a, j = b), so the data enter stage A and leave each stage in the result layout. An A
+// operand wants lane l to hold A[l % R][k] for eight consecutive k of its lane group; the move from the result
+// layout to the operand layout is a fixed exchange between lane groups with the same l % R, done in registers with
+// v_permlane32_swap (and, for R = 16, v_permlane16_swap): no LDS and no barrier inside a sequence.
+//
+// Split products (the accuracy of FP32 from 11- or 8-bit matrix inputs):
+//   FP16: per sequence and stage the operand is scaled by a power of two s (largest |value| near 2^14), split
+//         x = xh + xl, the table F = Fh + Fl, and D = (xh Fh + xh Fl + xl Fh) / s (three products);
+//   BF16: x = xh + xm + xl and F = Fh + Fm + Fl (bf16 keeps FP32's exponent range: no scaling), six products down
+//         to the 2^-24 level.
+// The DFT-matrix splits are built once per plan in FP64 on the device (k_mx_tables).
+//
+// Passes: PASS 0 transforms C columns of the four-step's n1 x n2 matrix (tile staged through LDS so global reads
+// are C*8-byte row runs) and multiplies by W_n^{col k}; PASS 1 transforms C rows (read straight into the register
+// layout) and stores them transposed, out[row + n1 k], through an LDS tile; PASS 2 is a whole transform of S
+// points per sequence (n = S), stored through LDS in point order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dft.hpp"
+#include "mfma.hpp"
+
+namespace crdfft
+{
+using namespace dev;
+
+namespace
+{
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256; // 4 waves per workgroup
+
+template <int R> struct Mx;
+
+template <> struct Mx<16>
+{
+    static constexpr int S = 256;
+    static constexpr int P = 4;      // complex values per lane
+    static constexpr int CHUNKS = 1; // K = 2R = 32 per product: one 16x16x32 instruction
+    static constexpr int C = 16;     // sequences per workgroup tile
+    // LDS pitch of a tile sequence (complex): 16 pad points per 64 so a wave's two 64-point halves land on
+    // different banks, and 2 more so the C-column row writes do not collide
+    static constexpr int SP = S + 16 * (S / 64) + 2;
+    using acc_t = floatx4;
+    __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+    __device__ static int pad(int o) { return o + 16 * (o >> 6); }
+    // K index of operand value v of chunk c in lane l
+    __device__ static int kidx(int l, int /*c*/, int v) { return 8 * (l >> 4) + v; }
+};
+
+template <> struct Mx<32>
+{
+    static constexpr int S = 1024;
+    static constexpr int P = 16;
+    static constexpr int CHUNKS = 4; // K = 2R = 64 per product: four 32x32x16 instructions
+    static constexpr int C = 8;
+    static constexpr int SP = S + 4;
+    using acc_t = floatx16;
+    __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+    __device__ static int pad(int o) { return o; }
+    __device__ static int kidx(int l, int c, int v) { return 16 * c + 8 * (l >> 5) + v; }
+};
+
+template <int R> constexpr int terms(int mode) { return mode == kSplitFp16 ? 2 : 3; }
+
+// Operand tables, per lane: [term][matrix re/im][chunk][lane][8] as float: matrix 0 = [Fr; -Fi] (gives the real
+// part), 1 = [Fi; Fr] (the imaginary part), F = W_R^{a k} (forward: Fi = -sin; the inverse negates every Fi).
+template <int R> __global__ void k_mx_tables(float* tab, int mode, int inverse)
+{
+    using M = Mx<R>;
+    const int l = threadIdx.x; // 64 lanes
+    const int j = l % R;
+    for (int c = 0; c < M::CHUNKS; ++c)
+    {
+        for (int v = 0; v < 8; ++v)
+        {
+            const int k = M::kidx(l, c, v);
+            const int a = k % R;
+            double sn, cs;
+            sincospi(2.0 * static_cast<double>((a * j) % R) / static_cast<double>(R), &sn, &cs);
+            const double fr = cs;
+            const double fi = inverse ? sn : -sn;
+            const double vals[2] = {k < R ? fr : -fi, k < R ? fi : fr};
+            for (int mtx = 0; mtx < 2; ++mtx)
+            {
+                const double x = vals[mtx];
+                float parts[3];
+                if (mode == kSplitFp16)
+                {
+                    const _Float16 h = static_cast<_Float16>(x);
+                    parts[0] = static_cast<float>(h);
+                    parts[1] = static_cast<float>(static_cast<_Float16>(x - static_cast<double>(h)));
+                    parts[2] = 0.0f;
+                }
+                else
+                {
+                    const __bf16 h = static_cast<__bf16>(static_cast<float>(x));
+                    const double r1 = x - static_cast<double>(static_cast<float>(h));
+                    const __bf16 m = static_cast<__bf16>(static_cast<float>(r1));
+                    const double r2 = r1 - static_cast<double>(static_cast<float>(m));
+                    parts[0] = static_cast<float>(h);
+                    parts[1] = static_cast<float>(m);
+                    parts[2] = static_cast<float>(static_cast<__bf16>(static_cast<float>(r2)));
+                }
+                for (int t = 0; t < terms<R>(mode); ++t)
+                {
+                    tab[((((t * 2 + mtx) * M::CHUNKS + c) * 64) + l) * 8 + v] = parts[t];
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+    {
+        v = fmaxf(v, __shfl_xor(v, o));
+    }
+    return v;
+}
+
+// Power of two s with max * s in [2^13, 2^14) (1 for an all-zero sequence or a non-finite max).
+__device__ __forceinline__ float fp16_scale(float maxabs)
+{
+    if (!(maxabs > 0.0f) || !(maxabs < 3.0e38f))
+    {
+        return 1.0f;
+    }
+    int e;
+    frexpf(maxabs, &e);
+    return ldexpf(1.0f, 14 - e);
+}
+
+__device__ __forceinline__ void swap32(float& x, float& y)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x), __builtin_bit_cast(uint32_t, y),
+                                                    false, false);
+    x = __builtin_bit_cast(float, static_cast<uint32_t>(r[0]));
+    y = __builtin_bit_cast(float, static_cast<uint32_t>(r[1]));
+}
+
+__device__ __forceinline__ void swap16(float& x, float& y)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, x), __builtin_bit_cast(uint32_t, y),
+                                                    false, false);
+    x = __builtin_bit_cast(float, static_cast<uint32_t>(r[0]));
+    y = __builtin_bit_cast(float, static_cast<uint32_t>(r[1]));
+}
+
+// Result layout (re[P], im[P]: i = row(l, r), j = l % R) -> A operand values op[c][v] (K index kidx(l, c, v): the
+// real parts of i = 0..R-1, then the imaginary parts). Consumes re and im.
+template <int R> __device__ __forceinline__ void to_operand(float (&re)[Mx<R>::P], float (&im)[Mx<R>::P],
+                                                            float (&op)[Mx<R>::CHUNKS][8])
+{
+    if constexpr (R == 16)
+    {
+        // group g = l/16 holds i = 4g + r; operand group g wants K = 8g..8g+7. After the 32-lane swap the rows of
+        // (re, im) are (R0, R1, I0, I1) and (R2, R3, I2, I3); after the 16-lane swap (R0, R2, I0, I2) and
+        // (R1, R3, I1, I3): group 0 holds Re i = 0..7, group 1 Re i = 8..15, groups 2, 3 the same of Im.
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            float x = re[r], y = im[r];
+            swap32(x, y);
+            swap16(x, y);
+            op[0][r] = x;
+            op[0][4 + r] = y;
+        }
+    }
+    else
+    {
+        // half h = l/32 holds i = 8(r/4) + 4h + r%4; chunk c (q = c % 2) wants i = 16q + 8h + v, v < 8: swapping the
+        // upper half of register block 2q with the lower half of block 2q + 1 puts i = 16q + 8h + (0..3) in block 2q
+        // and i = 16q + 8h + (4..7) in block 2q + 1 of half h.
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                float x = re[8 * q + j], y = re[8 * q + 4 + j];
+                swap32(x, y);
+                op[q][j] = x;
+                op[q][4 + j] = y;
+                float u = im[8 * q + j], w = im[8 * q + 4 + j];
+                swap32(u, w);
+                op[2 + q][j] = u;
+                op[2 + q][4 + j] = w;
+            }
+        }
+    }
+}
+
+template <int R> __device__ __forceinline__ floatx4 mfma(half8 a, half8 b, floatx4 acc)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+}
+template <int R> __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 acc)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+template <int R> __device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 acc)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+}
+template <int R> __device__ __forceinline__ floatx16 mfma(bf16x8 a, bf16x8 b, floatx16 acc)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+}
+
+template <int MODE> struct Elem;
+template <> struct Elem<kSplitFp16>
+{
+    using vec = half8;
+    using scalar = _Float16;
+    static constexpr int T = 2;
+};
+template <> struct Elem<kSplitBf16>
+{
+    using vec = bf16x8;
+    using scalar = __bf16;
+    static constexpr int T = 3;
+};
+
+// The B operands of this lane: [term][matrix][chunk], converted from the float tables once per kernel.
+template <int R, int MODE> struct Tables
+{
+    using V = typename Elem<MODE>::vec;
+    V b[Elem<MODE>::T][2][Mx<R>::CHUNKS];
+    __device__ __forceinline__ void load(const float* __restrict__ tab)
+    {
+        const int l = threadIdx.x & 63;
+#pragma unroll
+        for (int t = 0; t < Elem<MODE>::T; ++t)
+        {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+            {
+#pragma unroll
+                for (int c = 0; c < Mx<R>::CHUNKS; ++c)
+                {
+                    const float* p = tab + ((((t * 2 + m) * Mx<R>::CHUNKS + c) * 64) + l) * 8;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v)
+                    {
+                        b[t][m][c][v] = static_cast<typename Elem<MODE>::scalar>(p[v]);
+                    }
+                }
+            }
+        }
+    }
+};
+
+// One stage: re/im in the result layout (i = the index kept, j = l % R = the index transformed away next) in,
+// D_re/D_im of (re + i im)^T F out, in the same layout with j the new frequency index.
+template <int R, int MODE>
+__device__ __forceinline__ void stage(float (&re)[Mx<R>::P], float (&im)[Mx<R>::P], const Tables<R, MODE>& tb)
+{
+    using M = Mx<R>;
+    using E = Elem<MODE>;
+    float op[M::CHUNKS][8];
+    to_operand<R>(re, im, op);
+    float s = 1.0f;
+    if constexpr (MODE == kSplitFp16)
+    {
+        float m = 0.0f;
+#pragma unroll
+        for (int c = 0; c < M::CHUNKS; ++c)
+        {
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+            {
+                m = fmaxf(m, fabsf(op[c][v]));
+            }
+        }
+        s = fp16_scale(wave_max(m));
+    }
+    typename M::acc_t dre{}, dim{};
+#pragma unroll
+    for (int c = 0; c < M::CHUNKS; ++c)
+    {
+        typename E::vec xh, xm, xl;
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+        {
+            const float x = op[c][v] * s;
+            const typename E::scalar h = static_cast<typename E::scalar>(x);
+            const float r1 = x - static_cast<float>(h);
+            xh[v] = h;
+            if constexpr (MODE == kSplitFp16)
+            {
+                xl[v] = static_cast<typename E::scalar>(r1);
+            }
+            else
+            {
+                const typename E::scalar mm = static_cast<typename E::scalar>(r1);
+                xm[v] = mm;
+                xl[v] = static_cast<typename E::scalar>(r1 - static_cast<float>(mm));
+            }
+        }
+        if constexpr (MODE == kSplitFp16)
+        {
+            // small terms first, into accumulators still near zero
+            dre = mfma<R>(xl, tb.b[0][0][c], dre);
+            dre = mfma<R>(xh, tb.b[1][0][c], dre);
+            dre = mfma<R>(xh, tb.b[0][0][c], dre);
+            dim = mfma<R>(xl, tb.b[0][1][c], dim);
+            dim = mfma<R>(xh, tb.b[1][1][c], dim);
+            dim = mfma<R>(xh, tb.b[0][1][c], dim);
+        }
+        else
+        {
+            dre = mfma<R>(xh, tb.b[2][0][c], dre);
+            dre = mfma<R>(xm, tb.b[1][0][c], dre);
+            dre = mfma<R>(xl, tb.b[0][0][c], dre);
+            dre = mfma<R>(xh, tb.b[1][0][c], dre);
+            dre = mfma<R>(xm, tb.b[0][0][c], dre);
+            dre = mfma<R>(xh, tb.b[0][0][c], dre);
+            dim = mfma<R>(xh, tb.b[2][1][c], dim);
+            dim = mfma<R>(xm, tb.b[1][1][c], dim);
+            dim = mfma<R>(xl, tb.b[0][1][c], dim);
+            dim = mfma<R>(xh, tb.b[1][1][c], dim);
+            dim = mfma<R>(xm, tb.b[0][1][c], dim);
+            dim = mfma<R>(xh, tb.b[0][1][c], dim);
+        }
+    }
+    const float inv = 1.0f / s;
+#pragma unroll
+    for (int r = 0; r < M::P; ++r)
+    {
+        re[r] = dre[r] * inv;
+        im[r] = dim[r] * inv;
+    }
+}
+
+// The stage twiddles W_S^{b k1} of this lane (b = row(l, r), k1 = l % R), from the FP64-built table W_S^e:
+// R = 16 keeps all four; R = 32 keeps W^{(4h + j) k1} (j < 4) and W^{8 q k1} (q < 4) and multiplies them per use.
+template <int R> struct StageTw
+{
+    cf w[R == 16 ? 4 : 8];
+    __device__ __forceinline__ void load(const cf* __restrict__ tw)
+    {
+        const int l = threadIdx.x & 63;
+        const int k1 = l % R;
+        if constexpr (R == 16)
+        {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                w[r] = tw[(Mx<16>::row(l, r) * k1) & 255];
+            }
+        }
+        else
+        {
+            const int h = l >> 5;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                w[j] = tw[((4 * h + j) * k1) & 1023];
+                w[4 + j] = tw[(8 * j * k1) & 1023];
+            }
+        }
+    }
+    __device__ __forceinline__ cf at(int r) const
+    {
+        if constexpr (R == 16)
+        {
+            return w[r];
+        }
+        else
+        {
+            return cmul(w[r & 3], w[4 + (r >> 2)]);
+        }
+    }
+};
+
+// The whole sequence: re/im hold X[a = row(l, r)][b = l % R] on entry, X_out[k1 + R k2] at k1 = row(l, r),
+// k2 = l % R on exit.
+template <int R, int MODE, bool INV>
+__device__ __forceinline__ void sequence(float (&re)[Mx<R>::P], float (&im)[Mx<R>::P], const Tables<R, MODE>& tb,
+                                         const StageTw<R>& tw)
+{
+    stage<R, MODE>(re, im, tb); // now i = b, j = k1
+#pragma unroll
+    for (int r = 0; r < Mx<R>::P; ++r)
+    {
+        const cf y = ctw<INV>(cf{re[r], im[r]}, tw.at(r));
+        re[r] = y.x;
+        im[r] = y.y;
+    }
+    stage<R, MODE>(re, im, tb); // now i = k1, j = k2
+}
+
+template <int R, int PASS, int MODE, bool INV>
+__global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
+{
+    using M = Mx<R>;
+    constexpr int S = M::S, P = M::P, C = M::C;
+    extern __shared__ float lds_raw[];
+    cf* lds = reinterpret_cast<cf*>(lds_raw);
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wave = tid >> 6;
+    // PASS 0: a.n_seq = n2 (columns), PASS 1: a.n_seq = n1 (rows), PASS 2: a.n_seq = 1 and t counts transforms
+    const uint64_t groups = PASS == 2 ? 1 : a.n_seq / C;
+    const uint64_t t = PASS == 2 ? static_cast<uint64_t>(blockIdx.x) * C : blockIdx.x / groups;
+    const uint64_t g = PASS == 2 ? 0 : blockIdx.x - t * groups;
+    if (t >= a.count)
+    {
+        return; // uniform per workgroup
+    }
+    Tables<R, MODE> tb;
+    tb.load(a.tab);
+    StageTw<R> tw;
+    tw.load(reinterpret_cast<const cf*>(a.tws));
+    const int row_l = l % R;
+
+    if constexpr (PASS == 0)
+    {
+        // the tile: C columns (col0 + q) of the S-row column, element n of column c at n * ncols + c
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
+        const uint64_t ncols = a.n_seq;
+        const uint64_t col0 = g * C;
+        {
+            const int q = tid % C;
+#pragma unroll 4
+            for (int n = tid / C; n < S; n += kThreads / C)
+            {
+                lds[q * M::SP + M::pad(n)] = src[static_cast<uint64_t>(n) * ncols + col0 + q];
+            }
+        }
+        __syncthreads();
+        for (int q = wave; q < C; q += kThreads / 64)
+        {
+            float re[P], im[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                const cf v = lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)];
+                re[r] = v.x;
+                im[r] = v.y;
+            }
+            sequence<R, MODE, INV>(re, im, tb, tw);
+            // back in place, at o = R k1 + k2 (the store loop reads point k1 + R k2 from there)
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[r], im[r]};
+            }
+        }
+        __syncthreads();
+        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c; the threads of a half wave take
+        // points k2 apart by one so their LDS reads (o = R k1 + k2) fall on distinct banks
+        cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
+        const cf* thi = reinterpret_cast<const cf*>(a.thi);
+        const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+        const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
+        const int q = tid % C;
+        const uint64_t c = col0 + q;
+        constexpr int per = kThreads / C; // points per round
+#pragma unroll 4
+        for (int m = 0; m < S / per; ++m)
+        {
+            const int idx = tid / C + per * m; // idx = k2-major enumeration: k2 = idx % R, k1 = idx / R
+            const int k1 = idx / R;
+            const int k2 = idx % R;
+            const int k = k1 + R * k2;
+            cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
+            const uint64_t e = c * static_cast<uint64_t>(k);
+            v = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
+            dst[static_cast<uint64_t>(k) * ncols + c] = v;
+        }
+    }
+    else if constexpr (PASS == 1)
+    {
+        // rows row0 + q of the n1 x S matrix, read straight into the register layout (R consecutive points per
+        // lane group: 8R-byte runs)
+        const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
+        const uint64_t row0 = g * C;
+        for (int q = wave; q < C; q += kThreads / 64)
+        {
+            const cf* rp = src + (row0 + q) * S;
+            float re[P], im[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                const cf v = rp[R * M::row(l, r) + row_l];
+                re[r] = v.x;
+                im[r] = v.y;
+            }
+            sequence<R, MODE, INV>(re, im, tb, tw);
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[r], im[r]};
+            }
+        }
+        __syncthreads();
+        // point k of row row0 + q goes to out[row0 + q + n1 k]: C consecutive complex per k
+        cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
+        const uint64_t n1 = a.n_seq;
+        const int q = tid % C;
+        constexpr int per = kThreads / C;
+#pragma unroll 4
+        for (int m = 0; m < S / per; ++m)
+        {
+            const int idx = tid / C + per * m;
+            const int k1 = idx / R;
+            const int k2 = idx % R;
+            const int k = k1 + R * k2;
+            const cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
+            dst[row0 + q + n1 * static_cast<uint64_t>(k)] = cf{v.x * a.scale, v.y * a.scale};
+        }
+    }
+    else
+    {
+        // whole transforms t0 + q (n = S): straight loads; the result goes through the wave's LDS sequence so the
+        // store is in point order. Layout there: point k at k + k / R (the lanes' stride-R writes spread over the
+        // banks, the point-order reads stay nearly contiguous).
+        for (int q = wave; q < C; q += kThreads / 64)
+        {
+            const uint64_t tq = t + q;
+            if (tq >= a.count)
+            {
+                break; // uniform per wave
+            }
+            const cf* rp = reinterpret_cast<const cf*>(a.in) + tq * a.idist;
+            float re[P], im[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                const cf v = rp[R * M::row(l, r) + row_l];
+                re[r] = v.x;
+                im[r] = v.y;
+            }
+            sequence<R, MODE, INV>(re, im, tb, tw);
+            cf* ws = lds + q * M::SP;
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                const int k = M::row(l, r) + R * row_l;
+                ws[k + k / R] = cf{re[r] * a.scale, im[r] * a.scale};
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            cf* dst = reinterpret_cast<cf*>(a.out) + tq * a.odist;
+#pragma unroll
+            for (int m = 0; m < P; ++m)
+            {
+                const int k = l + 64 * m;
+                dst[k] = ws[k + k / R];
+            }
+        }
+    }
+}
+
+template <int R, int PASS, int MODE> hipError_t launch_one(bool inverse, const SplitArgs& a, hipStream_t s)
+{
+    using M = Mx<R>;
+    const uint64_t blocks = PASS == 2 ? (a.count + M::C - 1) / M::C : a.count * (a.n_seq / M::C);
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    if (blocks > 0x7FFFFFFFull || (PASS != 2 && a.n_seq % M::C != 0))
+    {
+        return hipErrorInvalidValue;
+    }
+    const unsigned lds = static_cast<unsigned>(M::C * M::SP * sizeof(cf));
+    auto kern = inverse ? k_mx<R, PASS, MODE, true> : k_mx<R, PASS, MODE, false>;
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (err != hipSuccess)
+    {
+        return err;
+    }
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int R, int PASS> hipError_t launch_mode(int mode, bool inverse, const SplitArgs& a, hipStream_t s)
+{
+    return mode == kSplitFp16 ? launch_one<R, PASS, kSplitFp16>(inverse, a, s)
+                              : launch_one<R, PASS, kSplitBf16>(inverse, a, s);
+}
+
+} // namespace
+
+bool split_radix_of(uint64_t len, int* radix)
+{
+    if (len == 256 || len == 1024)
+    {
+        *radix = len == 256 ? 16 : 32;
+        return true;
+    }
+    return false;
+}
+
+uint64_t split_table_floats(int radix)
+{
+    return static_cast<uint64_t>(3 * 2 * (radix == 16 ? 1 : 4) * 64 * 8);
+}
+
+hipError_t build_split_tables(float* tab, int radix, int mode, bool inverse, hipStream_t s)
+{
+    if (radix == 16)
+    {
+        hipLaunchKernelGGL(k_mx_tables<16>, dim3(1), dim3(64), 0, s, tab, mode, inverse ? 1 : 0);
+    }
+    else if (radix == 32)
+    {
+        hipLaunchKernelGGL(k_mx_tables<32>, dim3(1), dim3(64), 0, s, tab, mode, inverse ? 1 : 0);
+    }
+    else
+    {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_split_pass(int pass, int radix, int mode, bool inverse, const SplitArgs& a, hipStream_t s)
+{
+    if (radix == 16)
+    {
+        return pass == 0 ? launch_mode<16, 0>(mode, inverse, a, s)
+                         : (pass == 1 ? launch_mode<16, 1>(mode, inverse, a, s) : launch_mode<16, 2>(mode, inverse, a, s));
+    }
+    if (radix == 32)
+    {
+        return pass == 0 ? launch_mode<32, 0>(mode, inverse, a, s)
+                         : (pass == 1 ? launch_mode<32, 1>(mode, inverse, a, s) : launch_mode<32, 2>(mode, inverse, a, s));
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace crdfft

@@ -148,7 +148,9 @@ struct crdfft_plan_s
     DeviceBuffer em_tw1, em_tw2, em_thi, em_tlo;
     uint32_t em_hshift = 0;
     std::vector<std::pair<hipStream_t, std::unique_ptr<DeviceBuffer>>> em_work;
-    DeviceBuffer split_fwd, split_inv, tw256; // split-precision MFMA passes: operand splits, W_256
+    // split-precision MFMA passes: operand splits of each pass's radix per direction ([0] column or single pass,
+    // [1] row pass) and the stage twiddles W_S of each pass's sequence length
+    DeviceBuffer split_fwd[2], split_inv[2], split_tw[2];
     // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
     crdfft_plan inner = nullptr;
     crdfft_many_plan many = nullptr; // Bluestein plans of a 2/3/5/7-smooth n: the mixed-radix launch instead
@@ -252,19 +254,39 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
     const uint64_t cn = sh.complex_n;
     if (sh.precision != CRDFFT_FP32)
     {
-        const uint64_t fl = crdfft::split_table_floats();
-        CRDFFT_TRY(p->split_fwd.alloc(fl * sizeof(float)));
-        CRDFFT_TRY(p->split_inv.alloc(fl * sizeof(float)));
-        CRDFFT_TRY(crdfft::build_split_tables(p->split_fwd.as<float>(), sh.precision, false, s));
-        CRDFFT_TRY(crdfft::build_split_tables(p->split_inv.as<float>(), sh.precision, true, s));
-        CRDFFT_TRY(p->tw256.alloc(256 * sizeof(cfloat)));
-        CRDFFT_TRY(crdfft::build_roots(p->tw256.as<cfloat>(), 256, 1, 256, s));
-        p->hshift = 8;
-        CRDFFT_TRY(p->thi.alloc(256 * sizeof(cfloat)));
-        CRDFFT_TRY(p->tlo.alloc(256 * sizeof(cfloat)));
-        CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), 256, 256, cn, s));
-        CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), 256, 1, cn, s));
-        CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
+        const bool single = sh.algorithm == CRDFFT_ALGO_SINGLE;
+        const uint64_t lens[2] = {single ? cn : sh.n1, single ? 0 : sh.n2};
+        for (int ps = 0; ps < 2; ++ps)
+        {
+            int radix = 0;
+            if (lens[ps] == 0)
+            {
+                continue;
+            }
+            if (!crdfft::split_radix_of(lens[ps], &radix))
+            {
+                return CRDFFT_UNSUPPORTED;
+            }
+            const uint64_t fl = crdfft::split_table_floats(radix);
+            CRDFFT_TRY(p->split_fwd[ps].alloc(fl * sizeof(float)));
+            CRDFFT_TRY(p->split_inv[ps].alloc(fl * sizeof(float)));
+            CRDFFT_TRY(crdfft::build_split_tables(p->split_fwd[ps].as<float>(), radix, sh.precision, false, s));
+            CRDFFT_TRY(crdfft::build_split_tables(p->split_inv[ps].as<float>(), radix, sh.precision, true, s));
+            CRDFFT_TRY(p->split_tw[ps].alloc(lens[ps] * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_roots(p->split_tw[ps].as<cfloat>(), lens[ps], 1, lens[ps], s));
+        }
+        if (!single)
+        {
+            // the four-step twiddle W_n^e = hi[e >> hshift] * lo[e & (2^hshift - 1)]
+            const uint32_t lg = crdfft::log2_exact(cn);
+            p->hshift = lg / 2;
+            const uint64_t lo = uint64_t{1} << p->hshift, hi = cn >> p->hshift;
+            CRDFFT_TRY(p->thi.alloc(hi * sizeof(cfloat)));
+            CRDFFT_TRY(p->tlo.alloc(lo * sizeof(cfloat)));
+            CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), hi, lo, cn, s));
+            CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
+            CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
+        }
         return CRDFFT_OK;
     }
     if (sh.algorithm == CRDFFT_ALGO_SINGLE)
@@ -403,6 +425,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
         }
         case CRDFFT_ALGO_FOUR_STEP:
         {
+            const uint64_t chunk = sh.chunk;
             cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
             if (ws == nullptr)
             {
@@ -412,7 +435,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             // from and joined back to the caller's stream (profiled executes keep one stream: their events time
             // each launch on it)
             AuxPipe* aux = nullptr;
-            if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > sh.chunk && prof.acc == nullptr)
+            if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > chunk && prof.acc == nullptr)
             {
                 // a stream being captured cannot allocate its pipeline: without one it runs both halves in order
                 hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
@@ -431,10 +454,10 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             }
             const hipStream_t caller = s;
             const crdfft_status st = [&]() -> crdfft_status {
-                for (uint64_t c0 = 0; c0 < count; c0 += sh.chunk)
+                for (uint64_t c0 = 0; c0 < count; c0 += chunk)
                 {
-                    const uint64_t cnt = std::min(sh.chunk, count - c0);
-                    const bool second = aux != nullptr && ((c0 / sh.chunk) & 1) != 0;
+                    const uint64_t cnt = std::min(chunk, count - c0);
+                    const bool second = aux != nullptr && ((c0 / chunk) & 1) != 0;
                     const hipStream_t cs = second ? aux->stream : caller;
                     cfloat* const ws_c = second ? aux->ws.as<cfloat>() : ws;
                     crdfft::ColArgs ca{in + c0 * idist,
@@ -563,10 +586,27 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
     }
     if (sh.precision != CRDFFT_FP32 && !p->floor)
     {
-        // split-precision MFMA four-step of 256 x 256: column pass into the workspace, row pass out
+        // split-precision MFMA: one sequence per transform (n = 256, 1024), else the four-step's column pass into
+        // the workspace and its row pass out, per chunk
         const cfloat* x = static_cast<const cfloat*>(in);
         cfloat* y = static_cast<cfloat*>(out);
-        const float* tab = inverse ? p->split_inv.as<float>() : p->split_fwd.as<float>();
+        const float scale =
+            (inverse && (p->flags & CRDFFT_FLAG_NORMALIZE_INVERSE) != 0) ? 1.0f / static_cast<float>(sh.n) : 1.0f;
+        const float* tab0 = inverse ? p->split_inv[0].as<float>() : p->split_fwd[0].as<float>();
+        if (sh.algorithm == CRDFFT_ALGO_SINGLE)
+        {
+            if (x == y && p->idist != p->odist)
+            {
+                return CRDFFT_INVALID_ARGUMENT;
+            }
+            const crdfft::SplitArgs a{x,        y,          tab0,     p->split_tw[0].as<cfloat>(), nullptr, nullptr, 0,
+                                      sh.batch, p->idist,   p->odist, 1,                           scale};
+            CRDFFT_TRY(prof.before());
+            CRDFFT_TRY(crdfft::launch_split_pass(2, sh.e, sh.precision, inverse, a, s));
+            CRDFFT_TRY(prof.after(0, sh.complex_n * sh.batch));
+            return CRDFFT_OK;
+        }
+        const float* tab1 = inverse ? p->split_inv[1].as<float>() : p->split_fwd[1].as<float>();
         cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
         if (ws == nullptr)
         {
@@ -575,15 +615,15 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         for (uint64_t c0 = 0; c0 < sh.batch; c0 += sh.chunk)
         {
             const uint64_t cnt = std::min(sh.chunk, sh.batch - c0);
-            crdfft::SplitArgs a{x + c0 * p->idist, ws, tab, p->tw256.as<cfloat>(),
-                                p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, p->idist, sh.complex_n};
+            const crdfft::SplitArgs a{x + c0 * p->idist, ws, tab0, p->split_tw[0].as<cfloat>(), p->thi.as<cfloat>(),
+                                      p->tlo.as<cfloat>(), p->hshift, cnt, p->idist, sh.complex_n, sh.n2, 1.0f};
             CRDFFT_TRY(prof.before());
-            CRDFFT_TRY(crdfft::launch_split_pass(0, sh.precision, inverse, a, s));
+            CRDFFT_TRY(crdfft::launch_split_pass(0, sh.tile1, sh.precision, inverse, a, s));
             CRDFFT_TRY(prof.after(0, sh.complex_n * cnt));
-            crdfft::SplitArgs b{ws, y + c0 * p->odist, tab, p->tw256.as<cfloat>(),
-                                p->thi.as<cfloat>(), p->tlo.as<cfloat>(), p->hshift, cnt, sh.complex_n, p->odist};
+            const crdfft::SplitArgs b{ws, y + c0 * p->odist, tab1, p->split_tw[1].as<cfloat>(), nullptr, nullptr, 0,
+                                      cnt, sh.complex_n, p->odist, sh.n1, scale};
             CRDFFT_TRY(prof.before());
-            CRDFFT_TRY(crdfft::launch_split_pass(1, sh.precision, inverse, b, s));
+            CRDFFT_TRY(crdfft::launch_split_pass(1, sh.tile2, sh.precision, inverse, b, s));
             CRDFFT_TRY(prof.after(1, sh.complex_n * cnt));
         }
         return CRDFFT_OK;
@@ -797,7 +837,7 @@ crdfft_status create_plan(crdfft_plan* plan, uint64_t n, uint64_t batch, uint64_
     {
         rs = hip_status(hipStreamSynchronize(nullptr));
     }
-    if (rs == CRDFFT_OK && (flags & CRDFFT_FLAG_NO_SELF_CHECK) == 0)
+    if (rs == CRDFFT_OK && (flags & CRDFFT_FLAG_NO_SELF_CHECK) == 0 && shape.precision == CRDFFT_FP32)
     {
         // select_checked_kernel_level (kernel_dispatch.hpp:105-145): the selected kernels, else the floor
         rs = crdfft_self_check(p);
@@ -1232,7 +1272,8 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
     crdfft::shape_to_info(p->shape, info);
     info->check = p->check;
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
-                        p->tstep.bytes + p->tfull.bytes + p->wreal.bytes + p->split_fwd.bytes + p->split_inv.bytes + p->tw256.bytes;
+                        p->tstep.bytes + p->tfull.bytes + p->wreal.bytes + p->split_fwd[0].bytes + p->split_inv[0].bytes + p->split_tw[0].bytes +
+                        p->split_fwd[1].bytes + p->split_inv[1].bytes + p->split_tw[1].bytes;
     return CRDFFT_OK;
 }
 

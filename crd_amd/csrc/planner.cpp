@@ -191,16 +191,32 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
     }
     if (precision != CRDFFT_FP32)
     {
-        // the split-precision MFMA passes exist for the four-step of 256 x 256 (complex n = 2^16)
-        if (type != CRDFFT_C2C || cn != 65536)
+        // the split-precision MFMA passes run sequences of 256 (radix-16 stages) and 1024 points (radix-32): one
+        // sequence per transform at n = 256, 1024, else the four-step of n1 x n2 with n1, n2 in {256, 1024}
+        // (n = 2^16, 2^18, 2^20); tile1/tile2 hold the radices of the two passes
+        if (type != CRDFFT_C2C)
+        {
+            return CRDFFT_UNSUPPORTED;
+        }
+        if (cn == 256 || cn == 1024)
+        {
+            s.algorithm = CRDFFT_ALGO_SINGLE;
+            s.e = cn == 256 ? 16 : 32;
+            s.chunk = std::max<uint64_t>(batch, 1);
+            s.workspace_bytes = 0;
+            *out = s;
+            return CRDFFT_OK;
+        }
+        if (cn != (uint64_t{1} << 16) && cn != (uint64_t{1} << 18) && cn != (uint64_t{1} << 20))
         {
             return CRDFFT_UNSUPPORTED;
         }
         s.algorithm = CRDFFT_ALGO_FOUR_STEP;
-        s.n1 = 256;
-        s.n2 = 256;
-        s.e1 = s.e2 = 16;
-        s.tile1 = s.tile2 = 16;
+        s.n1 = cn == (uint64_t{1} << 20) ? 1024 : 256;
+        s.n2 = cn / s.n1;
+        s.e1 = s.e2 = 0;
+        s.tile1 = s.n1 == 256 ? 16 : 32;
+        s.tile2 = s.n2 == 256 ? 16 : 32;
         s.chunk = std::max<uint64_t>(1, kChunkBytes / (cn * 8));
         if (tuning != nullptr && tuning->chunk != 0)
         {

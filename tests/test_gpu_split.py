@@ -1,6 +1,8 @@
-"""Split-precision MFMA four-step (n = 2^16): radix-16 stages as DFT-matrix products on v_mfma_f32_16x16x32 in
-split BF16 (hi+mid+lo) and FP16 (hi+lo with per-sequence power-of-two scaling), FP32 accumulation, against the
-oracle's FP64 truth. The error bound is the tuned form's target, rel-L2 <= 1e-6 against FP64."""
+"""Split-precision MFMA transforms: radix-16 (256-point sequences, v_mfma_f32_16x16x32) and radix-32 (1024-point
+sequences, v_mfma_f32_32x32x16) stages as complex DFT-matrix products in split BF16 (hi+mid+lo) and FP16 (hi+lo
+with per-sequence power-of-two scaling), FP32 accumulation, the exchange between a sequence's two stages done with
+cross-lane swaps. Against the oracle's FP64 truth at the tuned form's target, rel-L2 <= 1e-6, for every length the
+planner takes: 256 and 1024 (one sequence per transform), 2^16 (256 x 256), 2^18 (256 x 1024), 2^20 (1024 x 1024)."""
 import numpy as np
 import pytest
 
@@ -9,11 +11,11 @@ pytestmark = pytest.mark.gpu
 N = 1 << 16
 
 
-def wide_dynamic_range(batch, seed):
+def wide_dynamic_range(batch, seed, n=N):
     """|x| log-uniform in [1e-4, 1e4], uniform phase (seeded)."""
     rng = np.random.default_rng(seed)
-    mag = 10.0 ** rng.uniform(-4, 4, (batch, N))
-    ph = rng.uniform(0, 2 * np.pi, (batch, N))
+    mag = 10.0 ** rng.uniform(-4, 4, (batch, n))
+    ph = rng.uniform(0, 2 * np.pi, (batch, n))
     return (mag * np.exp(1j * ph)).astype(np.complex64)
 
 
@@ -42,6 +44,30 @@ def test_split_precision_against_fp64(gpu, po, precision):
 
 
 @pytest.mark.parametrize("precision", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("n,batch", [(256, 37), (1024, 19), (1 << 18, 3), (1 << 20, 2)])
+def test_split_radix16_and_radix32_lengths(gpu, po, precision, n, batch):
+    """The radix-32 stages (1024-point sequences) and the mixed four-step against FP64, both directions, with odd
+    batches (a partial last workgroup of whole transforms at n = 256, 1024) and in place."""
+    x = wide_dynamic_range(batch, n ^ batch, n)
+    plan = gpu.FftPlan(n, batch, precision=precision)
+    info = plan.info()
+    assert info["precision"] != 0
+    for inverse in (False, True):
+        src = gpu.DeviceBuffer.from_array(x)
+        dst = gpu.DeviceBuffer(x.nbytes)
+        d = gpu.INVERSE if inverse else gpu.FORWARD
+        plan.execute(src, dst, d)
+        plan.execute(src, None, d)  # in place
+        gpu.synchronize()
+        ref = po.fft64_of_f32(x, inverse=inverse)
+        for buf in (dst, src):
+            y = buf.download(x.shape, np.complex64)
+            err = po.rel_l2(y, ref)
+            print(f"{precision} n={n} inverse={inverse} rel_l2={err:.3e}")
+            assert err <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["split_bf16", "split_fp16"])
 def test_split_precision_specials_and_batches(gpu, po, precision):
     # odd batch, in place, a zero transform and one with a single huge value
     batch = 3
@@ -60,10 +86,24 @@ def test_split_precision_specials_and_batches(gpu, po, precision):
     assert po.rel_l2(y[2], ref[2]) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [1024, 1 << 16])
+def test_split_normalized_inverse(gpu, po, n):
+    """CRDFFT_FLAG_NORMALIZE_INVERSE on a split plan: ifft_normalized's 1/n rides on the last store."""
+    from crd_amd import _native as N_
+
+    x = wide_dynamic_range(2, 3, n)
+    plan = gpu.FftPlan(n, 2, precision="split_fp16", flags=N_.FLAG_NORMALIZE_INVERSE)
+    buf = gpu.DeviceBuffer.from_array(x)
+    plan.execute(buf, None, gpu.INVERSE)
+    gpu.synchronize()
+    y = buf.download(x.shape, np.complex64)
+    assert po.rel_l2(y, po.fft64_of_f32(x, inverse=True) / n) <= 1e-6
+
+
 def test_split_precision_rejects_other_shapes(gpu):
     from crd_amd import _native as N_
 
-    for n, t in ((4096, "c2c"), (1 << 20, "c2c"), (N, "r2c")):
+    for n, t in ((4096, "c2c"), (1 << 17, "c2c"), (1 << 22, "c2c"), (N, "r2c")):
         with pytest.raises(N_.CrdfftError) as e:
             gpu.FftPlan(n, 2, t, precision="split_fp16")
         assert e.value.status == N_.UNSUPPORTED
