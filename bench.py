@@ -243,7 +243,33 @@ def bench_mixed(stream, Event):
     out["plain_per_length_bluestein"] = {"ms": round(ms, 4), "ffts_per_s": round(len(lengths) / (ms * 1e-3), 1),
                                          "gbs": round(16.0 * total / (ms * 1e-3) / 1e9, 1), "plans": len(plans),
                                          "launch_groups": len(calls)}
+    del plans, calls
+    # the same mixed-radix kernel and default radix plans, one launch per (length, group) in draw order: the
+    # algorithm without the single persistent launch
+    mplans = {}
+    mcalls = []
+    for n, first, cnt in groups:
+        key = (n, cnt)
+        if key not in mplans:
+            mplans[key] = many.ManyPlan(np.full(cnt, n, dtype=np.uint64))
+        mcalls.append((mplans[key], src.address + 8 * int(offs[first]), dst.address + 8 * int(offs[first])))
+
+    def per_length():
+        for p, a, b in mcalls:
+            p.execute(a, b, crd_amd.FORWARD, stream)
+
+    ms = time_loop(per_length, stream, 3, 1, Event)
+    out["per_length_launches_mixed_radix"] = {"ms": round(ms, 4), "ffts_per_s": round(len(lengths) / (ms * 1e-3), 1),
+                                              "plans": len(mplans), "launch_groups": len(mcalls)}
+    del mplans, mcalls
     out["tuned_vs_plain_ffts_per_s"] = round(out["plain_per_length_bluestein"]["ms"] / out["tuned"]["ms"], 2)
+    # where the factor comes from: the algorithm (mixed radix instead of chirp-z, both one launch per group), the
+    # single persistent launch (default radix plans), and the autotuner's radix plans
+    out["gain_breakdown"] = {
+        "algorithm": round(out["plain_per_length_bluestein"]["ms"] / out["per_length_launches_mixed_radix"]["ms"], 2),
+        "persistent_launch": round(out["per_length_launches_mixed_radix"]["ms"] / out["default_radix_plans"]["ms"], 2),
+        "tuning": round(out["default_radix_plans"]["ms"] / out["tuned"]["ms"], 3),
+    }
     return out
 
 
@@ -551,6 +577,16 @@ def main():
             extras["device_copy_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
             ms = time_loop(lambda: xout.copy_from(xin, stream=stream), stream, 5, 1, Event)
             extras["hip_memcpy_d2d_gbs"] = round(2.0 * xin.nbytes / (ms * 1e-3) / 1e9, 1)
+            # the same copy over working sets that fit the L2 (32 MiB), the 256 MB Infinity Cache (64-256 MiB) or
+            # neither: what a byte that leaves L2 costs when the Infinity Cache serves it instead of DRAM
+            sweep = []
+            for mib in (16, 32, 64, 128, 512, 2048):
+                nb = mib << 20
+                ms = time_loop(lambda: N.check(N.lib().crdfft_stream_copy(xout.ptr, xin.ptr, nb, stream.handle),
+                                               "crdfft_stream_copy"), stream, max(10, min(200, (4 << 30) // nb)),
+                               3, Event)
+                sweep.append({"set_mib": 2 * mib, "gbs": round(2.0 * nb / (ms * 1e-3) / 1e9, 1)})
+            extras["copy_by_working_set"] = sweep
         # R2C -> C2R round trip of both shapes
         rts = []
         for idx, (n, batch) in enumerate(SHAPES if "roundtrip" in want else []):

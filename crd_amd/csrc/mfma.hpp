@@ -18,7 +18,7 @@ struct SplitArgs
 {
     const cfloat* in;
     cfloat* out;
-    const float* tab;    // operand splits of the radix's DFT matrices (build_split_tables), this direction's
+    const uint16_t* tab; // operand splits of the radix's DFT matrices (build_split_tables), this direction's
     const cfloat* tws;   // W_S^e, e < S: the stage twiddles of the sequence length S
     const cfloat* thi;   // pass 0: four-step twiddle W_n^{j << hshift}
     const cfloat* tlo;   // pass 0: W_n^{j}, j < 2^hshift
@@ -32,8 +32,9 @@ struct SplitArgs
 
 // The radix of a sequence length (256 -> 16, 1024 -> 32), false for other lengths.
 bool split_radix_of(uint64_t len, int* radix);
-uint64_t split_table_floats(int radix);
-hipError_t build_split_tables(float* tab, int radix, int mode, bool inverse, hipStream_t s);
+// 16-bit entries of one direction's operand table (f16 or bf16 values by mode)
+uint64_t split_table_entries(int radix);
+hipError_t build_split_tables(uint16_t* tab, int radix, int mode, bool inverse, hipStream_t s);
 // pass 0: column pass with the four-step twiddle (in: the transforms, out: the intermediate, same layout);
 // pass 1: row pass with the transposed store (in: the intermediate, out: the transforms);
 // pass 2: whole transforms of one sequence each

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dft.hpp"
 #include "mfma.hpp"
 
@@ -55,7 +57,7 @@ template <> struct Mx<16>
     static constexpr int CHUNKS = 1; // K = 2R = 32 per product: one 16x16x32 instruction
     static constexpr int C = 16;     // sequences per workgroup tile
     // LDS pitch of a tile sequence (complex): 16 pad points per 64 so a wave's two 64-point halves land on
-    // different banks, and 2 more so the C-column row writes do not collide
+    // different banks, and 2 more so the C-column row writes do not collide (also >= S + S/R for PASS 2)
     static constexpr int SP = S + 16 * (S / 64) + 2;
     using acc_t = floatx4;
     __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
@@ -70,7 +72,8 @@ template <> struct Mx<32>
     static constexpr int P = 16;
     static constexpr int CHUNKS = 4; // K = 2R = 64 per product: four 32x32x16 instructions
     static constexpr int C = 8;
-    static constexpr int SP = S + 4;
+    // pitch: PASS 2 stages point k at k + k/R (up to S + S/R - 1); the +4 keeps C-column row writes apart
+    static constexpr int SP = S + S / 32 + 4;
     using acc_t = floatx16;
     __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
     __device__ static int pad(int o) { return o; }
@@ -78,10 +81,13 @@ template <> struct Mx<32>
 };
 
 template <int R> constexpr int terms(int mode) { return mode == kSplitFp16 ? 2 : 3; }
+static_assert(Mx<16>::SP >= Mx<16>::S + Mx<16>::S / 16 && Mx<32>::SP >= Mx<32>::S + Mx<32>::S / 32,
+              "the point-order staging of PASS 2 fits a sequence's pitch");
 
-// Operand tables, per lane: [term][matrix re/im][chunk][lane][8] as float: matrix 0 = [Fr; -Fi] (gives the real
-// part), 1 = [Fi; Fr] (the imaginary part), F = W_R^{a k} (forward: Fi = -sin; the inverse negates every Fi).
-template <int R> __global__ void k_mx_tables(float* tab, int mode, int inverse)
+// Operand tables, per lane: [term][matrix re/im][chunk][lane][8] 16-bit values (f16 or bf16 by mode, so a lane
+// loads its eight as one 128-bit access): matrix 0 = [Fr; -Fi] (gives the real part), 1 = [Fi; Fr] (the imaginary
+// part), F = W_R^{a k} (forward: Fi = -sin; the inverse negates every Fi).
+template <int R> __global__ void k_mx_tables(uint16_t* tab, int mode, int inverse)
 {
     using M = Mx<R>;
     const int l = threadIdx.x; // 64 lanes
@@ -100,13 +106,12 @@ template <int R> __global__ void k_mx_tables(float* tab, int mode, int inverse)
             for (int mtx = 0; mtx < 2; ++mtx)
             {
                 const double x = vals[mtx];
-                float parts[3];
+                uint16_t parts[3] = {0, 0, 0};
                 if (mode == kSplitFp16)
                 {
                     const _Float16 h = static_cast<_Float16>(x);
-                    parts[0] = static_cast<float>(h);
-                    parts[1] = static_cast<float>(static_cast<_Float16>(x - static_cast<double>(h)));
-                    parts[2] = 0.0f;
+                    parts[0] = __builtin_bit_cast(uint16_t, h);
+                    parts[1] = __builtin_bit_cast(uint16_t, static_cast<_Float16>(x - static_cast<double>(h)));
                 }
                 else
                 {
@@ -114,9 +119,9 @@ template <int R> __global__ void k_mx_tables(float* tab, int mode, int inverse)
                     const double r1 = x - static_cast<double>(static_cast<float>(h));
                     const __bf16 m = static_cast<__bf16>(static_cast<float>(r1));
                     const double r2 = r1 - static_cast<double>(static_cast<float>(m));
-                    parts[0] = static_cast<float>(h);
-                    parts[1] = static_cast<float>(m);
-                    parts[2] = static_cast<float>(static_cast<__bf16>(static_cast<float>(r2)));
+                    parts[0] = __builtin_bit_cast(uint16_t, h);
+                    parts[1] = __builtin_bit_cast(uint16_t, m);
+                    parts[2] = __builtin_bit_cast(uint16_t, static_cast<__bf16>(static_cast<float>(r2)));
                 }
                 for (int t = 0; t < terms<R>(mode); ++t)
                 {
@@ -125,16 +130,6 @@ template <int R> __global__ void k_mx_tables(float* tab, int mode, int inverse)
             }
         }
     }
-}
-
-__device__ __forceinline__ float wave_max(float v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-    {
-        v = fmaxf(v, __shfl_xor(v, o));
-    }
-    return v;
 }
 
 // Power of two s with max * s in [2^13, 2^14) (1 for an all-zero sequence or a non-finite max).
@@ -163,6 +158,27 @@ __device__ __forceinline__ void swap16(float& x, float& y)
                                                     false, false);
     x = __builtin_bit_cast(float, static_cast<uint32_t>(r[0]));
     y = __builtin_bit_cast(float, static_cast<uint32_t>(r[1]));
+}
+
+// Largest value over the wave, in every lane: DPP within each row of 16 (quad swaps, half-row and row mirrors),
+// then the row pairs and halves through the lane swaps (no LDS traffic).
+__device__ __forceinline__ float wave_max(float v)
+{
+    const auto dpp = [](float x, auto ctrl) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), decltype(ctrl)::value,
+                                                                      0xF, 0xF, false));
+    };
+    v = fmaxf(v, dpp(v, std::integral_constant<int, 0xB1>{})); // quad_perm [1,0,3,2]
+    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x4E>{})); // quad_perm [2,3,0,1]
+    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x141>{})); // row_half_mirror
+    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x140>{})); // row_mirror
+    float x = v, y = v;
+    swap16(x, y); // rows (0, 0, 2, 2) and (1, 1, 3, 3)
+    v = fmaxf(x, y);
+    x = v;
+    y = v;
+    swap32(x, y); // halves (lo, lo) and (hi, hi)
+    return fmaxf(x, y);
 }
 
 // Result layout (re[P], im[P]: i = row(l, r), j = l % R) -> A operand values op[c][v] (K index kidx(l, c, v): the
@@ -245,7 +261,7 @@ template <int R, int MODE> struct Tables
 {
     using V = typename Elem<MODE>::vec;
     V b[Elem<MODE>::T][2][Mx<R>::CHUNKS];
-    __device__ __forceinline__ void load(const float* __restrict__ tab)
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ tab)
     {
         const int l = threadIdx.x & 63;
 #pragma unroll
@@ -257,12 +273,7 @@ template <int R, int MODE> struct Tables
 #pragma unroll
                 for (int c = 0; c < Mx<R>::CHUNKS; ++c)
                 {
-                    const float* p = tab + ((((t * 2 + m) * Mx<R>::CHUNKS + c) * 64) + l) * 8;
-#pragma unroll
-                    for (int v = 0; v < 8; ++v)
-                    {
-                        b[t][m][c][v] = static_cast<typename Elem<MODE>::scalar>(p[v]);
-                    }
+                    b[t][m][c] = *reinterpret_cast<const V*>(tab + ((((t * 2 + m) * Mx<R>::CHUNKS + c) * 64) + l) * 8);
                 }
             }
         }
@@ -440,11 +451,20 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
         const uint64_t ncols = a.n_seq;
         const uint64_t col0 = g * C;
         {
+            // every load of the tile issued before any is waited for (S*C/kThreads per thread)
+            constexpr int L = S * C / kThreads;
             const int q = tid % C;
-#pragma unroll 4
-            for (int n = tid / C; n < S; n += kThreads / C)
+            const int n0 = tid / C;
+            cf v[L];
+#pragma unroll
+            for (int m = 0; m < L; ++m)
             {
-                lds[q * M::SP + M::pad(n)] = src[static_cast<uint64_t>(n) * ncols + col0 + q];
+                v[m] = src[static_cast<uint64_t>(n0 + (kThreads / C) * m) * ncols + col0 + q];
+            }
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+            {
+                lds[q * M::SP + M::pad(n0 + (kThreads / C) * m)] = v[m];
             }
         }
         __syncthreads();
@@ -495,16 +515,29 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
         // lane group: 8R-byte runs)
         const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
         const uint64_t row0 = g * C;
-        for (int q = wave; q < C; q += kThreads / 64)
+        // every sequence of this wave loaded before the first is transformed
+        constexpr int NQ = C / (kThreads / 64);
+        cf pre[NQ][P];
+#pragma unroll
+        for (int u = 0; u < NQ; ++u)
         {
-            const cf* rp = src + (row0 + q) * S;
+            const cf* rp = src + (row0 + wave + u * (kThreads / 64)) * S;
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                pre[u][r] = rp[R * M::row(l, r) + row_l];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; ++u)
+        {
+            const int q = wave + u * (kThreads / 64);
             float re[P], im[P];
 #pragma unroll
             for (int r = 0; r < P; ++r)
             {
-                const cf v = rp[R * M::row(l, r) + row_l];
-                re[r] = v.x;
-                im[r] = v.y;
+                re[r] = pre[u][r].x;
+                im[r] = pre[u][r].y;
             }
             sequence<R, MODE, INV>(re, im, tb, tw);
 #pragma unroll
@@ -535,21 +568,37 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
         // whole transforms t0 + q (n = S): straight loads; the result goes through the wave's LDS sequence so the
         // store is in point order. Layout there: point k at k + k / R (the lanes' stride-R writes spread over the
         // banks, the point-order reads stay nearly contiguous).
-        for (int q = wave; q < C; q += kThreads / 64)
+        constexpr int NQ = C / (kThreads / 64);
+        cf pre[NQ][P];
+#pragma unroll
+        for (int u = 0; u < NQ; ++u)
         {
+            const uint64_t tq = t + wave + u * (kThreads / 64);
+            if (tq < a.count) // uniform per wave
+            {
+                const cf* rp = reinterpret_cast<const cf*>(a.in) + tq * a.idist;
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    pre[u][r] = rp[R * M::row(l, r) + row_l];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; ++u)
+        {
+            const int q = wave + u * (kThreads / 64);
             const uint64_t tq = t + q;
             if (tq >= a.count)
             {
                 break; // uniform per wave
             }
-            const cf* rp = reinterpret_cast<const cf*>(a.in) + tq * a.idist;
             float re[P], im[P];
 #pragma unroll
             for (int r = 0; r < P; ++r)
             {
-                const cf v = rp[R * M::row(l, r) + row_l];
-                re[r] = v.x;
-                im[r] = v.y;
+                re[r] = pre[u][r].x;
+                im[r] = pre[u][r].y;
             }
             sequence<R, MODE, INV>(re, im, tb, tw);
             cf* ws = lds + q * M::SP;
@@ -615,12 +664,12 @@ bool split_radix_of(uint64_t len, int* radix)
     return false;
 }
 
-uint64_t split_table_floats(int radix)
+uint64_t split_table_entries(int radix)
 {
     return static_cast<uint64_t>(3 * 2 * (radix == 16 ? 1 : 4) * 64 * 8);
 }
 
-hipError_t build_split_tables(float* tab, int radix, int mode, bool inverse, hipStream_t s)
+hipError_t build_split_tables(uint16_t* tab, int radix, int mode, bool inverse, hipStream_t s)
 {
     if (radix == 16)
     {

@@ -267,11 +267,11 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
             {
                 return CRDFFT_UNSUPPORTED;
             }
-            const uint64_t fl = crdfft::split_table_floats(radix);
-            CRDFFT_TRY(p->split_fwd[ps].alloc(fl * sizeof(float)));
-            CRDFFT_TRY(p->split_inv[ps].alloc(fl * sizeof(float)));
-            CRDFFT_TRY(crdfft::build_split_tables(p->split_fwd[ps].as<float>(), radix, sh.precision, false, s));
-            CRDFFT_TRY(crdfft::build_split_tables(p->split_inv[ps].as<float>(), radix, sh.precision, true, s));
+            const uint64_t fl = crdfft::split_table_entries(radix);
+            CRDFFT_TRY(p->split_fwd[ps].alloc(fl * sizeof(uint16_t)));
+            CRDFFT_TRY(p->split_inv[ps].alloc(fl * sizeof(uint16_t)));
+            CRDFFT_TRY(crdfft::build_split_tables(p->split_fwd[ps].as<uint16_t>(), radix, sh.precision, false, s));
+            CRDFFT_TRY(crdfft::build_split_tables(p->split_inv[ps].as<uint16_t>(), radix, sh.precision, true, s));
             CRDFFT_TRY(p->split_tw[ps].alloc(lens[ps] * sizeof(cfloat)));
             CRDFFT_TRY(crdfft::build_roots(p->split_tw[ps].as<cfloat>(), lens[ps], 1, lens[ps], s));
         }
@@ -592,7 +592,7 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         cfloat* y = static_cast<cfloat*>(out);
         const float scale =
             (inverse && (p->flags & CRDFFT_FLAG_NORMALIZE_INVERSE) != 0) ? 1.0f / static_cast<float>(sh.n) : 1.0f;
-        const float* tab0 = inverse ? p->split_inv[0].as<float>() : p->split_fwd[0].as<float>();
+        const uint16_t* tab0 = inverse ? p->split_inv[0].as<uint16_t>() : p->split_fwd[0].as<uint16_t>();
         if (sh.algorithm == CRDFFT_ALGO_SINGLE)
         {
             if (x == y && p->idist != p->odist)
@@ -606,7 +606,7 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
             CRDFFT_TRY(prof.after(0, sh.complex_n * sh.batch));
             return CRDFFT_OK;
         }
-        const float* tab1 = inverse ? p->split_inv[1].as<float>() : p->split_fwd[1].as<float>();
+        const uint16_t* tab1 = inverse ? p->split_inv[1].as<uint16_t>() : p->split_fwd[1].as<uint16_t>();
         cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
         if (ws == nullptr)
         {
