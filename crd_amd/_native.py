@@ -153,6 +153,8 @@ _SIGS = {
     "crdfft_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "crdfft_execute_batched": (_i32, [_vp, _vp, _u64, _i32, _vp]),
     "crdfft_execute_reference": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "crdfft_execute_noalloc": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "crdfft_noalloc_scratch_bytes": (_u64, []),
     "crdfft_self_check": (_i32, [_vp]),
     "crdfft_execute_profiled": (_i32, [_vp, _vp, _vp, _i32, _vp, ctypes.POINTER(PhaseTimes)]),
     "crdfft_plan_get_info": (_i32, [_vp, ctypes.POINTER(PlanInfo)]),

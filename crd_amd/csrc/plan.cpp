@@ -425,8 +425,23 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
         }
         case CRDFFT_ALGO_FOUR_STEP:
         {
-            const uint64_t chunk = sh.chunk;
-            cfloat* ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
+            const crdfft::ScratchOverride* ov = crdfft::scratch_override();
+            uint64_t chunk = sh.chunk;
+            cfloat* ws = nullptr;
+            if (ov != nullptr)
+            {
+                // the caller-free form: the library's static scratch, in chunks that fit it, one pipeline
+                chunk = std::min<uint64_t>(sh.chunk, ov->bytes / (cn * sizeof(cfloat)));
+                if (chunk == 0)
+                {
+                    return CRDFFT_INVALID_SIZE;
+                }
+                ws = static_cast<cfloat*>(ov->ptr);
+            }
+            else
+            {
+                ws = static_cast<cfloat*>(p->workspace_of.get(p->workspace, s));
+            }
             if (ws == nullptr)
             {
                 return CRDFFT_OUT_OF_MEMORY;
@@ -435,7 +450,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
             // from and joined back to the caller's stream (profiled executes keep one stream: their events time
             // each launch on it)
             AuxPipe* aux = nullptr;
-            if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > chunk && prof.acc == nullptr)
+            if ((sh.hints & crdfft::kHintTwoPipes) != 0 && count > chunk && prof.acc == nullptr && ov == nullptr)
             {
                 // a stream being captured cannot allocate its pipeline: without one it runs both halves in order
                 hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
@@ -1087,6 +1102,13 @@ crdfft_status crdfft_execute_profiled(crdfft_plan p, const void* in, void* out, 
 
 namespace crdfft
 {
+namespace
+{
+thread_local const ScratchOverride* t_override = nullptr;
+}
+const ScratchOverride* scratch_override() { return t_override; }
+void set_scratch_override(const ScratchOverride* o) { t_override = o; }
+
 crdfft_status plan_use_floor(crdfft_plan p)
 {
     if (p == nullptr)

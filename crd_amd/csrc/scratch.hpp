@@ -119,4 +119,15 @@ struct PerStream
     }
 };
 
+// Caller-free scratch for one execute on this host thread (crdfft_execute_noalloc): while set, the four-step
+// passes use `ptr` (bytes long) as their intermediate, in chunks that fit it, on the caller's stream alone, and
+// every path that would allocate per-stream scratch refuses instead.
+struct ScratchOverride
+{
+    void* ptr;
+    uint64_t bytes;
+};
+const ScratchOverride* scratch_override();
+void set_scratch_override(const ScratchOverride* o);
+
 } // namespace crdfft

@@ -196,6 +196,18 @@ CRDFFT_API crdfft_status crdfft_execute_batched(crdfft_plan plan, void* data, ui
  * otherwise out of place. C2C plans only. */
 CRDFFT_API crdfft_status crdfft_execute_reference(crdfft_plan plan, const void* in, void* out, int32_t direction,
                                                   void* stream);
+/* crdfft_execute without any device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Instead of the
+ * plan's per-stream workspace (which a first execute on a new stream allocates) the four-step passes use the
+ * library's static device scratch, crdfft_noalloc_scratch_bytes() (64 MiB) per device, in chunks that fit it, on
+ * the caller's stream alone. Every call on one device shares that scratch: calls that overlap on one device race,
+ * so the caller serializes them. Plain FP32 plans of one kernel or the four-step only (C2C, R2C, C2R):
+ * CRDFFT_UNSUPPORTED for the fused, cooperative, split-precision and Bluestein forms; CRDFFT_INVALID_SIZE when one
+ * transform's four-step intermediate (8 * complex_n bytes) exceeds the scratch. Same results, bit for bit, as
+ * crdfft_execute on the same plan. */
+CRDFFT_API crdfft_status crdfft_execute_noalloc(crdfft_plan plan, const void* in, void* out, int32_t direction,
+                                                void* stream);
+CRDFFT_API uint64_t crdfft_noalloc_scratch_bytes(void);
 /* Known-answer check of the plan's kernels (synchronous; allocates its own small buffers). */
 CRDFFT_API crdfft_status crdfft_self_check(crdfft_plan plan);
 
