@@ -22,6 +22,7 @@ struct SplitArgs
     const cfloat* tws;   // W_S^e, e < S: the stage twiddles of the sequence length S
     const cfloat* thi;   // pass 0: four-step twiddle W_n^{j << hshift}
     const cfloat* tlo;   // pass 0: W_n^{j}, j < 2^hshift
+    const cfloat* tfull; // pass 0, when set: W_n^{c k} at [k * n2 + c] (one coalesced load per value, n <= 2^18)
     uint32_t hshift;
     uint64_t count;      // transforms (pass 2) or four-step matrices (passes 0, 1)
     uint64_t idist;

@@ -56,12 +56,15 @@ template <> struct Mx<16>
     static constexpr int P = 4;      // complex values per lane
     static constexpr int CHUNKS = 1; // K = 2R = 32 per product: one 16x16x32 instruction
     static constexpr int C = 16;     // sequences per workgroup tile
-    // LDS pitch of a tile sequence (complex): 16 pad points per 64 so a wave's two 64-point halves land on
-    // different banks, and 2 more so the C-column row writes do not collide (also >= S + S/R for PASS 2)
-    static constexpr int SP = S + 16 * (S / 64) + 2;
+    // LDS pitch of a tile sequence (complex): 2 spare points so the C-column row writes do not collide; PASS 2
+    // stages point k at k + k/R, so its pitch holds S + S/R
+    static constexpr int SP = S + 2;
+    static constexpr int SP2 = S + S / 16;
     using acc_t = floatx4;
     __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
-    __device__ static int pad(int o) { return o + 16 * (o >> 6); }
+    // the 16-point blocks of every odd 64-point block swapped in pairs, so a wave's two 64-point halves (lane groups
+    // 0 and 1 read points 64g + 16r + c) land on different banks without growing the tile
+    __device__ static int pad(int o) { return o ^ (((o >> 6) & 1) << 4); }
     // K index of operand value v of chunk c in lane l
     __device__ static int kidx(int l, int /*c*/, int v) { return 8 * (l >> 4) + v; }
 };
@@ -74,6 +77,7 @@ template <> struct Mx<32>
     static constexpr int C = 8;
     // pitch: PASS 2 stages point k at k + k/R (up to S + S/R - 1); the +4 keeps C-column row writes apart
     static constexpr int SP = S + S / 32 + 4;
+    static constexpr int SP2 = SP;
     using acc_t = floatx16;
     __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
     __device__ static int pad(int o) { return o; }
@@ -81,7 +85,7 @@ template <> struct Mx<32>
 };
 
 template <int R> constexpr int terms(int mode) { return mode == kSplitFp16 ? 2 : 3; }
-static_assert(Mx<16>::SP >= Mx<16>::S + Mx<16>::S / 16 && Mx<32>::SP >= Mx<32>::S + Mx<32>::S / 32,
+static_assert(Mx<16>::SP2 >= Mx<16>::S + Mx<16>::S / 16 && Mx<32>::SP2 >= Mx<32>::S + Mx<32>::S / 32,
               "the point-order staging of PASS 2 fits a sequence's pitch");
 
 // Operand tables, per lane: [term][matrix re/im][chunk][lane][8] 16-bit values (f16 or bf16 by mode, so a lane
@@ -132,16 +136,17 @@ template <int R> __global__ void k_mx_tables(uint16_t* tab, int mode, int invers
     }
 }
 
-// Power of two s with max * s in [2^13, 2^14) (1 for an all-zero sequence or a non-finite max).
-__device__ __forceinline__ float fp16_scale(float maxabs)
+// The exponent x of the power of two s = 2^x with max * s in [2^13, 2^14) (0 for an all-zero sequence or a
+// non-finite max): s and its exact reciprocal are ldexpf(1, x) and ldexpf(1, -x).
+__device__ __forceinline__ int fp16_scale_exp(float maxabs)
 {
     if (!(maxabs > 0.0f) || !(maxabs < 3.0e38f))
     {
-        return 1.0f;
+        return 0;
     }
     int e;
     frexpf(maxabs, &e);
-    return ldexpf(1.0f, 14 - e);
+    return 14 - e;
 }
 
 __device__ __forceinline__ void swap32(float& x, float& y)
@@ -290,6 +295,7 @@ __device__ __forceinline__ void stage(float (&re)[Mx<R>::P], float (&im)[Mx<R>::
     float op[M::CHUNKS][8];
     to_operand<R>(re, im, op);
     float s = 1.0f;
+    int sx = 0;
     if constexpr (MODE == kSplitFp16)
     {
         float m = 0.0f;
@@ -302,7 +308,8 @@ __device__ __forceinline__ void stage(float (&re)[Mx<R>::P], float (&im)[Mx<R>::
                 m = fmaxf(m, fabsf(op[c][v]));
             }
         }
-        s = fp16_scale(wave_max(m));
+        sx = fp16_scale_exp(wave_max(m));
+        s = ldexpf(1.0f, sx);
     }
     typename M::acc_t dre{}, dim{};
 #pragma unroll
@@ -353,7 +360,7 @@ __device__ __forceinline__ void stage(float (&re)[Mx<R>::P], float (&im)[Mx<R>::
             dim = mfma<R>(xh, tb.b[0][1][c], dim);
         }
     }
-    const float inv = 1.0f / s;
+    const float inv = ldexpf(1.0f, -sx); // exact: s is a power of two
 #pragma unroll
     for (int r = 0; r < M::P; ++r)
     {
@@ -504,8 +511,15 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
             const int k2 = idx % R;
             const int k = k1 + R * k2;
             cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
-            const uint64_t e = c * static_cast<uint64_t>(k);
-            v = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
+            if (a.tfull != nullptr)
+            {
+                v = ctw<INV>(v, reinterpret_cast<const cf*>(a.tfull)[static_cast<uint64_t>(k) * ncols + c]);
+            }
+            else
+            {
+                const uint64_t e = c * static_cast<uint64_t>(k);
+                v = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
+            }
             dst[static_cast<uint64_t>(k) * ncols + c] = v;
         }
     }
@@ -601,7 +615,7 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
                 im[r] = pre[u][r].y;
             }
             sequence<R, MODE, INV>(re, im, tb, tw);
-            cf* ws = lds + q * M::SP;
+            cf* ws = lds + q * M::SP2;
 #pragma unroll
             for (int r = 0; r < P; ++r)
             {
@@ -634,7 +648,7 @@ template <int R, int PASS, int MODE> hipError_t launch_one(bool inverse, const S
     {
         return hipErrorInvalidValue;
     }
-    const unsigned lds = static_cast<unsigned>(M::C * M::SP * sizeof(cf));
+    const unsigned lds = static_cast<unsigned>(M::C * (PASS == 2 ? M::SP2 : M::SP) * sizeof(cf));
     auto kern = inverse ? k_mx<R, PASS, MODE, true> : k_mx<R, PASS, MODE, false>;
     hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));

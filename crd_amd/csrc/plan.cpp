@@ -285,6 +285,11 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
             CRDFFT_TRY(p->tlo.alloc(lo * sizeof(cfloat)));
             CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), hi, lo, cn, s));
             CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
+            if (cn <= crdfft::kFullTwiddleMax)
+            {
+                CRDFFT_TRY(p->tfull.alloc(cn * sizeof(cfloat)));
+                CRDFFT_TRY(crdfft::build_full_twiddle(p->tfull.as<cfloat>(), sh.n1, sh.n2, cn, s));
+            }
             CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
         }
         return CRDFFT_OK;
@@ -614,8 +619,8 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
             {
                 return CRDFFT_INVALID_ARGUMENT;
             }
-            const crdfft::SplitArgs a{x,        y,          tab0,     p->split_tw[0].as<cfloat>(), nullptr, nullptr, 0,
-                                      sh.batch, p->idist,   p->odist, 1,                           scale};
+            const crdfft::SplitArgs a{x,       y,        tab0,     p->split_tw[0].as<cfloat>(), nullptr, nullptr, nullptr,
+                                      0,       sh.batch, p->idist, p->odist,                    1,       scale};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(2, sh.e, sh.precision, inverse, a, s));
             CRDFFT_TRY(prof.after(0, sh.complex_n * sh.batch));
@@ -631,12 +636,13 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         {
             const uint64_t cnt = std::min(sh.chunk, sh.batch - c0);
             const crdfft::SplitArgs a{x + c0 * p->idist, ws, tab0, p->split_tw[0].as<cfloat>(), p->thi.as<cfloat>(),
-                                      p->tlo.as<cfloat>(), p->hshift, cnt, p->idist, sh.complex_n, sh.n2, 1.0f};
+                                      p->tlo.as<cfloat>(), p->tfull.as<cfloat>(), p->hshift, cnt, p->idist,
+                                      sh.complex_n, sh.n2, 1.0f};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(0, sh.tile1, sh.precision, inverse, a, s));
             CRDFFT_TRY(prof.after(0, sh.complex_n * cnt));
-            const crdfft::SplitArgs b{ws, y + c0 * p->odist, tab1, p->split_tw[1].as<cfloat>(), nullptr, nullptr, 0,
-                                      cnt, sh.complex_n, p->odist, sh.n1, scale};
+            const crdfft::SplitArgs b{ws,     y + c0 * p->odist, tab1, p->split_tw[1].as<cfloat>(), nullptr, nullptr,
+                                      nullptr, 0, cnt, sh.complex_n, p->odist, sh.n1, scale};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(1, sh.tile2, sh.precision, inverse, b, s));
             CRDFFT_TRY(prof.after(1, sh.complex_n * cnt));
