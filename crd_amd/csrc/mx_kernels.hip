@@ -285,87 +285,132 @@ template <int R, int MODE> struct Tables
     }
 };
 
-// One stage: re/im in the result layout (i = the index kept, j = l % R = the index transformed away next) in,
-// D_re/D_im of (re + i im)^T F out, in the same layout with j the new frequency index.
-template <int R, int MODE>
-__device__ __forceinline__ void stage(float (&re)[Mx<R>::P], float (&im)[Mx<R>::P], const Tables<R, MODE>& tb)
+// One stage of NS independent sequences at once (their products interleave, so one sequence's matrix-core latency
+// hides behind the others'): re/im in the result layout (i = the index kept, j = l % R = the index transformed away
+// next) in, D_re/D_im of (re + i im)^T F out, in the same layout with j the new frequency index.
+template <int R, int MODE, int NS>
+__device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS][Mx<R>::P], const Tables<R, MODE>& tb)
 {
     using M = Mx<R>;
     using E = Elem<MODE>;
-    float op[M::CHUNKS][8];
-    to_operand<R>(re, im, op);
-    float s = 1.0f;
-    int sx = 0;
-    if constexpr (MODE == kSplitFp16)
+    float op[NS][M::CHUNKS][8];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
     {
-        float m = 0.0f;
-#pragma unroll
-        for (int c = 0; c < M::CHUNKS; ++c)
-        {
-#pragma unroll
-            for (int v = 0; v < 8; ++v)
-            {
-                m = fmaxf(m, fabsf(op[c][v]));
-            }
-        }
-        sx = fp16_scale_exp(wave_max(m));
-        s = ldexpf(1.0f, sx);
+        to_operand<R>(re[u], im[u], op[u]);
     }
-    typename M::acc_t dre{}, dim{};
+    int sx[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+    {
+        sx[u] = 0;
+        if constexpr (MODE == kSplitFp16)
+        {
+            float m = 0.0f;
+#pragma unroll
+            for (int c = 0; c < M::CHUNKS; ++c)
+            {
+#pragma unroll
+                for (int v = 0; v < 8; ++v)
+                {
+                    m = fmaxf(m, fabsf(op[u][c][v]));
+                }
+            }
+            sx[u] = fp16_scale_exp(wave_max(m));
+        }
+    }
+    typename M::acc_t dre[NS], dim[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+    {
+        dre[u] = typename M::acc_t{};
+        dim[u] = typename M::acc_t{};
+    }
 #pragma unroll
     for (int c = 0; c < M::CHUNKS; ++c)
     {
-        typename E::vec xh, xm, xl;
+        typename E::vec xh[NS], xm[NS], xl[NS];
 #pragma unroll
-        for (int v = 0; v < 8; ++v)
+        for (int u = 0; u < NS; ++u)
         {
-            const float x = op[c][v] * s;
-            const typename E::scalar h = static_cast<typename E::scalar>(x);
-            const float r1 = x - static_cast<float>(h);
-            xh[v] = h;
-            if constexpr (MODE == kSplitFp16)
+            const float su = MODE == kSplitFp16 ? ldexpf(1.0f, sx[u]) : 1.0f;
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
             {
-                xl[v] = static_cast<typename E::scalar>(r1);
-            }
-            else
-            {
-                const typename E::scalar mm = static_cast<typename E::scalar>(r1);
-                xm[v] = mm;
-                xl[v] = static_cast<typename E::scalar>(r1 - static_cast<float>(mm));
+                const float x = op[u][c][v] * su;
+                const typename E::scalar h = static_cast<typename E::scalar>(x);
+                const float r1 = x - static_cast<float>(h);
+                xh[u][v] = h;
+                if constexpr (MODE == kSplitFp16)
+                {
+                    xl[u][v] = static_cast<typename E::scalar>(r1);
+                }
+                else
+                {
+                    const typename E::scalar mm = static_cast<typename E::scalar>(r1);
+                    xm[u][v] = mm;
+                    xl[u][v] = static_cast<typename E::scalar>(r1 - static_cast<float>(mm));
+                }
             }
         }
+        // small terms first, into accumulators still near zero; sequences innermost, so consecutive products are
+        // independent
         if constexpr (MODE == kSplitFp16)
         {
-            // small terms first, into accumulators still near zero
-            dre = mfma<R>(xl, tb.b[0][0][c], dre);
-            dre = mfma<R>(xh, tb.b[1][0][c], dre);
-            dre = mfma<R>(xh, tb.b[0][0][c], dre);
-            dim = mfma<R>(xl, tb.b[0][1][c], dim);
-            dim = mfma<R>(xh, tb.b[1][1][c], dim);
-            dim = mfma<R>(xh, tb.b[0][1][c], dim);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+            {
+#pragma unroll
+                for (int u = 0; u < NS; ++u)
+                {
+                    typename M::acc_t& d = mt == 0 ? dre[u] : dim[u];
+                    d = mfma<R>(xl[u], tb.b[0][mt][c], d);
+                }
+#pragma unroll
+                for (int u = 0; u < NS; ++u)
+                {
+                    typename M::acc_t& d = mt == 0 ? dre[u] : dim[u];
+                    d = mfma<R>(xh[u], tb.b[1][mt][c], d);
+                }
+#pragma unroll
+                for (int u = 0; u < NS; ++u)
+                {
+                    typename M::acc_t& d = mt == 0 ? dre[u] : dim[u];
+                    d = mfma<R>(xh[u], tb.b[0][mt][c], d);
+                }
+            }
         }
         else
         {
-            dre = mfma<R>(xh, tb.b[2][0][c], dre);
-            dre = mfma<R>(xm, tb.b[1][0][c], dre);
-            dre = mfma<R>(xl, tb.b[0][0][c], dre);
-            dre = mfma<R>(xh, tb.b[1][0][c], dre);
-            dre = mfma<R>(xm, tb.b[0][0][c], dre);
-            dre = mfma<R>(xh, tb.b[0][0][c], dre);
-            dim = mfma<R>(xh, tb.b[2][1][c], dim);
-            dim = mfma<R>(xm, tb.b[1][1][c], dim);
-            dim = mfma<R>(xl, tb.b[0][1][c], dim);
-            dim = mfma<R>(xh, tb.b[1][1][c], dim);
-            dim = mfma<R>(xm, tb.b[0][1][c], dim);
-            dim = mfma<R>(xh, tb.b[0][1][c], dim);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+            {
+#pragma unroll
+                for (int pr = 0; pr < 6; ++pr)
+                {
+#pragma unroll
+                    for (int u = 0; u < NS; ++u)
+                    {
+                        typename M::acc_t& d = mt == 0 ? dre[u] : dim[u];
+                        // (data term, table term): (h, l), (m, m), (l, h), (h, m), (m, h), (h, h)
+                        const typename E::vec& x = pr == 1 || pr == 4 ? xm[u] : (pr == 2 ? xl[u] : xh[u]);
+                        const int tt = pr == 0 ? 2 : (pr == 1 || pr == 3 ? 1 : 0);
+                        d = mfma<R>(x, tb.b[tt][mt][c], d);
+                    }
+                }
+            }
         }
     }
-    const float inv = ldexpf(1.0f, -sx); // exact: s is a power of two
 #pragma unroll
-    for (int r = 0; r < M::P; ++r)
+    for (int u = 0; u < NS; ++u)
     {
-        re[r] = dre[r] * inv;
-        im[r] = dim[r] * inv;
+        const float inv = ldexpf(1.0f, -sx[u]); // exact: the scale is a power of two
+#pragma unroll
+        for (int r = 0; r < M::P; ++r)
+        {
+            re[u][r] = dre[u][r] * inv;
+            im[u][r] = dim[u][r] * inv;
+        }
     }
 }
 
@@ -410,21 +455,25 @@ template <int R> struct StageTw
     }
 };
 
-// The whole sequence: re/im hold X[a = row(l, r)][b = l % R] on entry, X_out[k1 + R k2] at k1 = row(l, r),
+// NS whole sequences: re/im hold X[a = row(l, r)][b = l % R] on entry, X_out[k1 + R k2] at k1 = row(l, r),
 // k2 = l % R on exit.
-template <int R, int MODE, bool INV>
-__device__ __forceinline__ void sequence(float (&re)[Mx<R>::P], float (&im)[Mx<R>::P], const Tables<R, MODE>& tb,
-                                         const StageTw<R>& tw)
+template <int R, int MODE, bool INV, int NS>
+__device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)[NS][Mx<R>::P],
+                                          const Tables<R, MODE>& tb, const StageTw<R>& tw)
 {
-    stage<R, MODE>(re, im, tb); // now i = b, j = k1
+    stage<R, MODE, NS>(re, im, tb); // now i = b, j = k1
 #pragma unroll
-    for (int r = 0; r < Mx<R>::P; ++r)
+    for (int u = 0; u < NS; ++u)
     {
-        const cf y = ctw<INV>(cf{re[r], im[r]}, tw.at(r));
-        re[r] = y.x;
-        im[r] = y.y;
+#pragma unroll
+        for (int r = 0; r < Mx<R>::P; ++r)
+        {
+            const cf y = ctw<INV>(cf{re[u][r], im[u][r]}, tw.at(r));
+            re[u][r] = y.x;
+            im[u][r] = y.y;
+        }
     }
-    stage<R, MODE>(re, im, tb); // now i = k1, j = k2
+    stage<R, MODE, NS>(re, im, tb); // now i = k1, j = k2
 }
 
 template <int R, int PASS, int MODE, bool INV>
@@ -432,6 +481,7 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
 {
     using M = Mx<R>;
     constexpr int S = M::S, P = M::P, C = M::C;
+    constexpr int NS = R == 16 ? 2 : 1; // sequences transformed together (interleaved products)
     extern __shared__ float lds_raw[];
     cf* lds = reinterpret_cast<cf*>(lds_raw);
     const int tid = threadIdx.x;
@@ -475,22 +525,34 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
             }
         }
         __syncthreads();
-        for (int q = wave; q < C; q += kThreads / 64)
-        {
-            float re[P], im[P];
+        constexpr int NQ = C / (kThreads / 64);
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+        for (int u0 = 0; u0 < NQ; u0 += NS)
+        {
+            float re[NS][P], im[NS][P];
+#pragma unroll
+            for (int w = 0; w < NS; ++w)
             {
-                const cf v = lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)];
-                re[r] = v.x;
-                im[r] = v.y;
+                const int q = wave + (u0 + w) * (kThreads / 64);
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    const cf v = lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)];
+                    re[w][r] = v.x;
+                    im[w][r] = v.y;
+                }
             }
-            sequence<R, MODE, INV>(re, im, tb, tw);
+            sequences<R, MODE, INV, NS>(re, im, tb, tw);
             // back in place, at o = R k1 + k2 (the store loop reads point k1 + R k2 from there)
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+            for (int w = 0; w < NS; ++w)
             {
-                lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[r], im[r]};
+                const int q = wave + (u0 + w) * (kThreads / 64);
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[w][r], im[w][r]};
+                }
             }
         }
         __syncthreads();
@@ -543,21 +605,29 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
             }
         }
 #pragma unroll
-        for (int u = 0; u < NQ; ++u)
+        for (int u0 = 0; u0 < NQ; u0 += NS)
         {
-            const int q = wave + u * (kThreads / 64);
-            float re[P], im[P];
+            float re[NS][P], im[NS][P];
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+            for (int w = 0; w < NS; ++w)
             {
-                re[r] = pre[u][r].x;
-                im[r] = pre[u][r].y;
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    re[w][r] = pre[u0 + w][r].x;
+                    im[w][r] = pre[u0 + w][r].y;
+                }
             }
-            sequence<R, MODE, INV>(re, im, tb, tw);
+            sequences<R, MODE, INV, NS>(re, im, tb, tw);
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+            for (int w = 0; w < NS; ++w)
             {
-                lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[r], im[r]};
+                const int q = wave + (u0 + w) * (kThreads / 64);
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    lds[q * M::SP + M::pad(R * M::row(l, r) + row_l)] = cf{re[w][r], im[w][r]};
+                }
             }
         }
         __syncthreads();
@@ -588,6 +658,11 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
         for (int u = 0; u < NQ; ++u)
         {
             const uint64_t tq = t + wave + u * (kThreads / 64);
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+            {
+                pre[u][r] = cf{0.0f, 0.0f};
+            }
             if (tq < a.count) // uniform per wave
             {
                 const cf* rp = reinterpret_cast<const cf*>(a.in) + tq * a.idist;
@@ -599,38 +674,50 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
             }
         }
 #pragma unroll
-        for (int u = 0; u < NQ; ++u)
+        for (int u0 = 0; u0 < NQ; u0 += NS)
         {
-            const int q = wave + u * (kThreads / 64);
-            const uint64_t tq = t + q;
-            if (tq >= a.count)
+            if (t + wave + u0 * (kThreads / 64) >= a.count)
             {
-                break; // uniform per wave
+                break; // uniform per wave: no sequence of this group exists
             }
-            float re[P], im[P];
+            float re[NS][P], im[NS][P];
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+            for (int w = 0; w < NS; ++w)
             {
-                re[r] = pre[u][r].x;
-                im[r] = pre[u][r].y;
+#pragma unroll
+                for (int r = 0; r < P; ++r)
+                {
+                    re[w][r] = pre[u0 + w][r].x;
+                    im[w][r] = pre[u0 + w][r].y;
+                }
             }
-            sequence<R, MODE, INV>(re, im, tb, tw);
-            cf* ws = lds + q * M::SP2;
+            sequences<R, MODE, INV, NS>(re, im, tb, tw);
 #pragma unroll
-            for (int r = 0; r < P; ++r)
+            for (int w = 0; w < NS; ++w)
             {
-                const int k = M::row(l, r) + R * row_l;
-                ws[k + k / R] = cf{re[r] * a.scale, im[r] * a.scale};
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            cf* dst = reinterpret_cast<cf*>(a.out) + tq * a.odist;
+                const int q = wave + (u0 + w) * (kThreads / 64);
+                const uint64_t tq = t + q;
+                if (tq >= a.count)
+                {
+                    break; // uniform per wave: the group's last sequences past the batch are not stored
+                }
+                cf* ws = lds + q * M::SP2;
 #pragma unroll
-            for (int m = 0; m < P; ++m)
-            {
-                const int k = l + 64 * m;
-                dst[k] = ws[k + k / R];
+                for (int r = 0; r < P; ++r)
+                {
+                    const int k = M::row(l, r) + R * row_l;
+                    ws[k + k / R] = cf{re[w][r] * a.scale, im[w][r] * a.scale};
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                cf* dst = reinterpret_cast<cf*>(a.out) + tq * a.odist;
+#pragma unroll
+                for (int m = 0; m < P; ++m)
+                {
+                    const int k = l + 64 * m;
+                    dst[k] = ws[k + k / R];
+                }
             }
         }
     }
