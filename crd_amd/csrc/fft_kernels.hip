@@ -308,7 +308,13 @@ __global__ __launch_bounds__((ColCfg<N1, E, CC>::threads)) void k_cols(ColArgs a
             v[m] = ctw<INV>(v[m], w);
         }
     }
-    store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + off, C::T * ncols, 1.0f,
+    // tiled intermediate: row slot m of column group g is the contiguous run [g*T*CC + (m*ncols/CC)*T*CC, +T*CC),
+    // point (i, c) at tid
+    const uint32_t soff = (a.hints & kHintTiled) != 0
+                              ? static_cast<uint32_t>(lb - t * groups) * static_cast<uint32_t>(C::T * CC) +
+                                    static_cast<uint32_t>(tid)
+                              : off;
+    store_strided<E>(v, reinterpret_cast<cf*>(a.out) + t * a.odist + soff, C::T * ncols, 1.0f,
                      (a.hints & kHintNtStore) != 0);
 }
 
@@ -517,8 +523,13 @@ __global__ __launch_bounds__((RowTrCfg<N2, E, RR>::threads)) void k_rowtr(RowTrA
     cf v[E];
     if (active)
     {
-        load_strided<E>(v, reinterpret_cast<const cf*>(a.in) + t * a.idist + (g * RR + r) * N2 + i,
-                        static_cast<uint32_t>(C::T), (a.hints & kHintNtLoad) != 0);
+        // tiled intermediate: rows g*RR .. g*RR+RR-1 are one contiguous block, slot m of every row a run of RR*T
+        // points (row r, column i at tid)
+        const bool tiled = (a.hints & kHintTiled) != 0;
+        load_strided<E>(v,
+                        reinterpret_cast<const cf*>(a.in) + t * a.idist +
+                            (tiled ? g * RR * N2 + static_cast<uint64_t>(tid) : (g * RR + r) * N2 + i),
+                        static_cast<uint32_t>(tiled ? C::threads : C::T), (a.hints & kHintNtLoad) != 0);
     }
     else
     {

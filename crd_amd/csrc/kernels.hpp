@@ -35,6 +35,12 @@ constexpr uint32_t kHintNtStore = 2u;
 // Plan knob beside the cache hints: run the four-step's column pass as a persistent launch that prefetches its next
 // tile (k_cols_pf), where that kernel is instantiated.
 constexpr uint32_t kHintPersistent = 8u;
+// Launch-time flag of a C2C four-step whose column pass has T1 = n1/e1 threads per column equal to the row pass's
+// rows per workgroup and tile1 columns equal to its n2/e2 threads per row: the intermediate is laid out in tiles of
+// T1 rows x tile1 columns, ordered [k1 / T1][n2 / tile1], so the column pass stores each of its row slots as one run
+// of T1*tile1 points and the row pass reads its rows as runs of as many (instead of tile1-point row segments at the
+// matrix's row stride). Not a plan hint: the executor sets it on both passes' arguments.
+constexpr uint32_t kHintTiled = 64u;
 
 // Column transforms of length n1 over `ncols` adjacent columns of a row-major n1 x ncols matrix, for `count`
 // matrices at t*dist: four-step pass 1 (with the factored twiddle W_N^{n2 k1} when thi != nullptr) and the

@@ -473,6 +473,14 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                 }
             }
             const hipStream_t caller = s;
+            // tiled intermediate (kHintTiled) where the column pass's threads per column equal the row pass's rows
+            // per workgroup and its columns per workgroup the row pass's threads per row (n = 2^16: 16 and 16)
+            const bool plain_pair = c2r_w == nullptr && r2c_w == nullptr && (sh.hints & crdfft::kHintPersistent) == 0;
+            const uint32_t tiled =
+                plain_pair && sh.e1 > 0 && sh.e2 > 0 && static_cast<int>(sh.n1 / sh.e1) == sh.tile2 &&
+                        static_cast<int>(sh.n2 / sh.e2) == sh.tile1 && sh.n2 % static_cast<uint64_t>(sh.tile1) == 0
+                    ? crdfft::kHintTiled
+                    : 0u;
             const crdfft_status st = [&]() -> crdfft_status {
                 for (uint64_t c0 = 0; c0 < count; c0 += chunk)
                 {
@@ -492,7 +500,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                                        cnt,
                                        idist,
                                        cn,
-                                       sh.hints & crdfft::kHintNtLoad,
+                                       (sh.hints & crdfft::kHintNtLoad) | tiled,
                                        c2r_w,
                                        1.0f / static_cast<float>(cn)};
                     CRDFFT_TRY(prof.before());
@@ -510,7 +518,7 @@ crdfft_status run_complex(crdfft_plan_s* p, const cfloat* in, cfloat* out, uint6
                     }
                     CRDFFT_TRY(prof.after(0, cn * cnt));
                     crdfft::RowTrArgs ra{ws_c, out + c0 * odist, p->tw2.as<cfloat>(), sh.n1, cnt,
-                                         cn, odist, scale, sh.hints & crdfft::kHintNtStore, r2c_w};
+                                         cn, odist, scale, (sh.hints & crdfft::kHintNtStore) | tiled, r2c_w};
                     CRDFFT_TRY(prof.before());
                     if (r2c_w != nullptr)
                     {
