@@ -137,7 +137,8 @@ template <int R> __global__ void k_mx_tables(uint16_t* tab, int mode, int invers
 }
 
 // The exponent x of the power of two s = 2^x with max * s in [2^13, 2^14) (0 for an all-zero sequence or a
-// non-finite max): s and its exact reciprocal are ldexpf(1, x) and ldexpf(1, -x).
+// non-finite max; at most 126, so s stays a finite float for sequences below 2^-112, which then keep a smaller
+// scaled maximum).
 __device__ __forceinline__ int fp16_scale_exp(float maxabs)
 {
     if (!(maxabs > 0.0f) || !(maxabs < 3.0e38f))
@@ -146,7 +147,7 @@ __device__ __forceinline__ int fp16_scale_exp(float maxabs)
     }
     int e;
     frexpf(maxabs, &e);
-    return 14 - e;
+    return min(14 - e, 126);
 }
 
 __device__ __forceinline__ void swap32(float& x, float& y)
@@ -165,25 +166,25 @@ __device__ __forceinline__ void swap16(float& x, float& y)
     y = __builtin_bit_cast(float, static_cast<uint32_t>(r[1]));
 }
 
-// Largest value over the wave, in every lane: DPP within each row of 16 (quad swaps, half-row and row mirrors),
-// then the row pairs and halves through the lane swaps (no LDS traffic).
+// Largest value over the wave, in every lane, of values >= 0: their bit patterns order as the values do (and a
+// NaN above every finite value), so the reduction runs on integers: DPP within each row of 16 (quad swaps, half-row
+// and row mirrors), then the row pairs and halves through the lane swaps (no LDS traffic).
 __device__ __forceinline__ float wave_max(float v)
 {
-    const auto dpp = [](float x, auto ctrl) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), decltype(ctrl)::value,
-                                                                      0xF, 0xF, false));
+    const auto dpp = [](uint32_t x, auto ctrl) {
+        return static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(x), decltype(ctrl)::value, 0xF, 0xF, true));
     };
-    v = fmaxf(v, dpp(v, std::integral_constant<int, 0xB1>{})); // quad_perm [1,0,3,2]
-    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x4E>{})); // quad_perm [2,3,0,1]
-    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x141>{})); // row_half_mirror
-    v = fmaxf(v, dpp(v, std::integral_constant<int, 0x140>{})); // row_mirror
-    float x = v, y = v;
-    swap16(x, y); // rows (0, 0, 2, 2) and (1, 1, 3, 3)
-    v = fmaxf(x, y);
-    x = v;
-    y = v;
-    swap32(x, y); // halves (lo, lo) and (hi, hi)
-    return fmaxf(x, y);
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    u = max(u, dpp(u, std::integral_constant<int, 0xB1>{}));  // quad_perm [1,0,3,2]
+    u = max(u, dpp(u, std::integral_constant<int, 0x4E>{}));  // quad_perm [2,3,0,1]
+    u = max(u, dpp(u, std::integral_constant<int, 0x141>{})); // row_half_mirror
+    u = max(u, dpp(u, std::integral_constant<int, 0x140>{})); // row_mirror
+    auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false); // rows (0, 0, 2, 2) and (1, 1, 3, 3)
+    u = max(static_cast<uint32_t>(r[0]), static_cast<uint32_t>(r[1]));
+    r = __builtin_amdgcn_permlane32_swap(u, u, false, false); // halves (lo, lo) and (hi, hi)
+    u = max(static_cast<uint32_t>(r[0]), static_cast<uint32_t>(r[1]));
+    return __builtin_bit_cast(float, u);
 }
 
 // Result layout (re[P], im[P]: i = row(l, r), j = l % R) -> A operand values op[c][v] (K index kidx(l, c, v): the
@@ -287,9 +288,14 @@ template <int R, int MODE> struct Tables
 
 // One stage of NS independent sequences at once (their products interleave, so one sequence's matrix-core latency
 // hides behind the others'): re/im in the result layout (i = the index kept, j = l % R = the index transformed away
-// next) in, D_re/D_im of (re + i im)^T F out, in the same layout with j the new frequency index.
-template <int R, int MODE, int NS>
-__device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS][Mx<R>::P], const Tables<R, MODE>& tb)
+// next) in, D_re/D_im of (re + i im)^T F out, in the same layout with j the new frequency index. carry[u] is the
+// power-of-two exponent sequence u's values already carry (FP16: each stage scales its operands by 2^sx): with
+// UNSCALE the outputs are brought back to scale 1 (ldexp by -(carry + sx), exact) and carry is reset, else the
+// outputs keep the factor and carry grows by sx (the next stage's scan and scale see the scaled values, so the
+// factor costs nothing until the last stage removes it).
+template <int R, int MODE, int NS, bool UNSCALE>
+__device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS][Mx<R>::P], int (&carry)[NS],
+                                      const Tables<R, MODE>& tb)
 {
     using M = Mx<R>;
     using E = Elem<MODE>;
@@ -404,12 +410,26 @@ __device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS]
 #pragma unroll
     for (int u = 0; u < NS; ++u)
     {
-        const float inv = ldexpf(1.0f, -sx[u]); // exact: the scale is a power of two
-#pragma unroll
-        for (int r = 0; r < M::P; ++r)
+        if constexpr (UNSCALE && MODE == kSplitFp16)
         {
-            re[u][r] = dre[u][r] * inv;
-            im[u][r] = dim[u][r] * inv;
+            const int back = -(carry[u] + sx[u]);
+#pragma unroll
+            for (int r = 0; r < M::P; ++r)
+            {
+                re[u][r] = ldexpf(dre[u][r], back);
+                im[u][r] = ldexpf(dim[u][r], back);
+            }
+            carry[u] = 0;
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < M::P; ++r)
+            {
+                re[u][r] = dre[u][r];
+                im[u][r] = dim[u][r];
+            }
+            carry[u] += sx[u];
         }
     }
 }
@@ -461,7 +481,13 @@ template <int R, int MODE, bool INV, int NS>
 __device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)[NS][Mx<R>::P],
                                           const Tables<R, MODE>& tb, const StageTw<R>& tw)
 {
-    stage<R, MODE, NS>(re, im, tb); // now i = b, j = k1
+    int carry[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+    {
+        carry[u] = 0;
+    }
+    stage<R, MODE, NS, false>(re, im, carry, tb); // now i = b, j = k1 (values times 2^carry)
 #pragma unroll
     for (int u = 0; u < NS; ++u)
     {
@@ -473,11 +499,13 @@ __device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)
             im[u][r] = y.y;
         }
     }
-    stage<R, MODE, NS>(re, im, tb); // now i = k1, j = k2
+    stage<R, MODE, NS, true>(re, im, carry, tb); // now i = k1, j = k2, at scale 1
 }
 
+// R = 16: the LDS tile allows four workgroups per CU (four waves per SIMD), so the registers are held to what four
+// waves per SIMD can have (128)
 template <int R, int PASS, int MODE, bool INV>
-__global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 16 ? 4 : 1))) void k_mx(SplitArgs a)
 {
     using M = Mx<R>;
     constexpr int S = M::S, P = M::P, C = M::C;
@@ -507,6 +535,14 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
         const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
         const uint64_t ncols = a.n_seq;
         const uint64_t col0 = g * C;
+        // store round m of thread tid writes point k = k1 + R k2 of column q = tid % C, where idx = tid / C + m *
+        // kThreads / C enumerates k2-major (k2 = idx % R, k1 = idx / R): the threads of a half wave take points k2
+        // apart by one, so their LDS reads (o = R k1 + k2) fall on distinct banks
+        constexpr int kStores = S * C / kThreads;
+        const auto store_k = [](int tid_, int m) {
+            const int idx = tid_ / C + (kThreads / C) * m;
+            return idx / R + R * (idx % R);
+        };
         {
             // every load of the tile issued before any is waited for (S*C/kThreads per thread)
             constexpr int L = S * C / kThreads;
@@ -556,33 +592,40 @@ __global__ __launch_bounds__(kThreads) void k_mx(SplitArgs a)
             }
         }
         __syncthreads();
-        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c; the threads of a half wave take
-        // points k2 apart by one so their LDS reads (o = R k1 + k2) fall on distinct banks
+        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c
         cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
-        const cf* thi = reinterpret_cast<const cf*>(a.thi);
-        const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
-        const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
         const int q = tid % C;
         const uint64_t c = col0 + q;
-        constexpr int per = kThreads / C; // points per round
-#pragma unroll 4
-        for (int m = 0; m < S / per; ++m)
+        if (a.tfull != nullptr)
         {
-            const int idx = tid / C + per * m; // idx = k2-major enumeration: k2 = idx % R, k1 = idx / R
-            const int k1 = idx / R;
-            const int k2 = idx % R;
-            const int k = k1 + R * k2;
-            cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
-            if (a.tfull != nullptr)
+            // every twiddle load issued before the first is waited for
+            cf twp[kStores];
+#pragma unroll
+            for (int m = 0; m < kStores; ++m)
             {
-                v = ctw<INV>(v, reinterpret_cast<const cf*>(a.tfull)[static_cast<uint64_t>(k) * ncols + c]);
+                twp[m] = reinterpret_cast<const cf*>(a.tfull)[static_cast<uint64_t>(store_k(tid, m)) * ncols + c];
             }
-            else
+#pragma unroll
+            for (int m = 0; m < kStores; ++m)
             {
+                const int k = store_k(tid, m);
+                const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
+                dst[static_cast<uint64_t>(k) * ncols + c] = ctw<INV>(v, twp[m]);
+            }
+        }
+        else
+        {
+            const cf* thi = reinterpret_cast<const cf*>(a.thi);
+            const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+            const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
+#pragma unroll 4
+            for (int m = 0; m < kStores; ++m)
+            {
+                const int k = store_k(tid, m);
+                const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
                 const uint64_t e = c * static_cast<uint64_t>(k);
-                v = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
+                dst[static_cast<uint64_t>(k) * ncols + c] = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
             }
-            dst[static_cast<uint64_t>(k) * ncols + c] = v;
         }
     }
     else if constexpr (PASS == 1)
