@@ -346,46 +346,55 @@ def bench_r2r(stream, Event):
 
 def bench_split(stream, Event):
     """n = 2^16, batch 2048, |x| log-uniform in [1e-4, 1e4] with uniform phase: plain FP32 against the split-BF16
-    and split-FP16 MFMA forms, time and rel-L2 against FP64 (numpy, float64) on a sample."""
+    and split-FP16 MFMA forms, time and rel-L2 against FP64 (numpy, float64) on a sample. The same for every other
+    length the split forms take (256 and 1024: one radix-16 / radix-32 sequence per transform; 2^18 = 256 x 1024 and
+    2^20 = 1024 x 1024: radix-32 stages in one or both passes), each at 2^27 points per batch."""
     import crd_amd
+    from crd_amd import _native as N
 
-    n, batch, seed_rows = 1 << 16, 2048, 64
+    precs = ("fp32", "split_bf16", "split_fp16")
+    points = 1 << 27
+    src = crd_amd.DeviceBuffer(8 * points)
+    dst = crd_amd.DeviceBuffer(8 * points)
+    seed_pts = 1 << 22
     rng = np.random.default_rng(4)
-    mag = 10.0 ** rng.uniform(-4, 4, (seed_rows, n))
-    x = (mag * np.exp(1j * rng.uniform(0, 2 * np.pi, (seed_rows, n)))).astype(np.complex64)
-    src = crd_amd.DeviceBuffer(8 * n * batch)
-    dst = crd_amd.DeviceBuffer(8 * n * batch)
+    mag = 10.0 ** rng.uniform(-4, 4, seed_pts)
+    x = (mag * np.exp(1j * rng.uniform(0, 2 * np.pi, seed_pts))).astype(np.complex64)
     src.upload(x)
-    for r in range(seed_rows, batch, seed_rows):  # the 64 seeded rows repeated over the batch
-        from crd_amd import _native as N
-
-        N.check(N.lib().crdfft_memcpy_d2d(src.address + 8 * n * r, src.ptr, 8 * n * seed_rows, stream.handle),
+    for off in range(seed_pts, points, seed_pts):  # the seeded 2^22 points repeated over the buffer
+        N.check(N.lib().crdfft_memcpy_d2d(src.address + 8 * off, src.ptr, 8 * seed_pts, stream.handle),
                 "crdfft_memcpy_d2d")
     stream.synchronize()
-    ref = np.fft.fft(x[:4].astype(np.complex128), axis=1)
-    out = {"n": n, "batch": batch}
-    precs = ("fp32", "split_bf16", "split_fp16")
-    plans = {prec: crd_amd.FftPlan(n, batch, precision=prec) for prec in precs}
-    # the three forms timed in alternation after ~0.2 s of warm executes each, so none of them is measured while the
-    # clocks recover from the host-side reference work above; the median of 5 rounds of 8 executes
-    for prec in precs:
-        time_loop(lambda: plans[prec].execute(src, dst, crd_amd.FORWARD, stream), stream, 200, 0, Event)
-    times = {prec: [] for prec in precs}
-    for _ in range(5):
+
+    def one_length(n, sample):
+        batch = points // n
+        ref = np.fft.fft(x[:sample * n].reshape(sample, n).astype(np.complex128), axis=1)
+        plans = {prec: crd_amd.FftPlan(n, batch, precision=prec) for prec in precs}
+        # the three forms timed in alternation after ~0.2 s of warm executes each, so none of them is measured while
+        # the clocks recover from host-side work; the median of 5 rounds of 8 executes
         for prec in precs:
-            times[prec].append(time_loop(lambda: plans[prec].execute(src, dst, crd_amd.FORWARD, stream), stream, 8, 1,
-                                         Event))
-    for prec in precs:
-        plans[prec].execute(src, dst, crd_amd.FORWARD, stream)
-        stream.synchronize()
-        y = dst.download((4, n), np.complex64)
-        ms = float(np.median(times[prec]))
-        out[prec] = {"ms": round(ms, 4), "ffts_per_s": round(batch / (ms * 1e-3), 1),
-                     "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
-                     "rel_l2_vs_fp64": rel_l2(y.astype(np.complex128), ref)}
-    del plans
-    out["split_bf16_vs_fp32_ffts_per_s"] = round(out["fp32"]["ms"] / out["split_bf16"]["ms"], 3)
-    out["split_fp16_vs_fp32_ffts_per_s"] = round(out["fp32"]["ms"] / out["split_fp16"]["ms"], 3)
+            time_loop(lambda: plans[prec].execute(src, dst, crd_amd.FORWARD, stream), stream, 200, 0, Event)
+        times = {prec: [] for prec in precs}
+        for _ in range(5):
+            for prec in precs:
+                times[prec].append(time_loop(lambda: plans[prec].execute(src, dst, crd_amd.FORWARD, stream), stream,
+                                             8, 1, Event))
+        res = {"n": n, "batch": batch}
+        for prec in precs:
+            plans[prec].execute(src, dst, crd_amd.FORWARD, stream)
+            stream.synchronize()
+            y = dst.download((sample, n), np.complex64)
+            ms = float(np.median(times[prec]))
+            res[prec] = {"ms": round(ms, 4), "ffts_per_s": round(batch / (ms * 1e-3), 1),
+                         "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
+                         "rel_l2_vs_fp64": rel_l2(y.astype(np.complex128), ref)}
+        del plans
+        res["split_bf16_vs_fp32_ffts_per_s"] = round(res["fp32"]["ms"] / res["split_bf16"]["ms"], 3)
+        res["split_fp16_vs_fp32_ffts_per_s"] = round(res["fp32"]["ms"] / res["split_fp16"]["ms"], 3)
+        return res
+
+    out = one_length(1 << 16, 4)
+    out["other_lengths"] = [one_length(n, s) for n, s in ((256, 64), (1024, 64), (1 << 18, 2), (1 << 20, 2))]
     return out
 
 
