@@ -160,6 +160,7 @@ def bench_c2c4096(stream, Event):
     in the 256 MB Infinity Cache; 16 B per point algorithmic; one single-pass kernel per execute."""
     import crd_amd
     from crd_amd import DeviceBuffer, FftPlan
+    from crd_amd import _native as N
 
     n, batch, nb = 4096, 1024, 4
     p = FftPlan(n, batch)
@@ -175,9 +176,21 @@ def bench_c2c4096(stream, Event):
         p.execute(ins[i], outs[i], crd_amd.FORWARD, stream)
 
     ms = time_loop(rot, stream, 40, 10, Event)
+
+    # the ceiling for this size: the streaming copy (16-byte elements, one read and one write per byte) over the same
+    # rotating buffer pairs, 64 MiB per launch
+    def rot_copy():
+        i = state["i"] % nb
+        state["i"] += 1
+        N.check(N.lib().crdfft_stream_copy(outs[i].ptr, ins[i].ptr, 8 * n * batch, stream.handle),
+                "crdfft_stream_copy")
+
+    copy_ms = time_loop(rot_copy, stream, 40, 10, Event)
     return {"ms": round(ms, 5), "gbs": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
             "frac_of_peak": round(16.0 * n * batch / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb}
+            "ffts_per_s": round(batch / (ms * 1e-3), 1), "rotating_buffers": nb,
+            "copy_same_bytes": {"ms": round(copy_ms, 5), "gbs": round(16.0 * n * batch / (copy_ms * 1e-3) / 1e9, 1)},
+            "vs_copy": round(copy_ms / ms, 3)}
 
 
 def bench_mixed(stream, Event):
