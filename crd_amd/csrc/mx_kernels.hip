@@ -43,6 +43,8 @@ namespace
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -339,23 +341,42 @@ __device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS]
 #pragma unroll
         for (int u = 0; u < NS; ++u)
         {
-            const float su = MODE == kSplitFp16 ? ldexpf(1.0f, sx[u]) : 1.0f;
-#pragma unroll
-            for (int v = 0; v < 8; ++v)
+            if constexpr (MODE == kSplitFp16)
             {
-                const float x = op[u][c][v] * su;
-                const typename E::scalar h = static_cast<typename E::scalar>(x);
-                const float r1 = x - static_cast<float>(h);
-                xh[u][v] = h;
-                if constexpr (MODE == kSplitFp16)
+                const float su = ldexpf(1.0f, sx[u]);
+#pragma unroll
+                for (int v = 0; v < 8; ++v)
                 {
-                    xl[u][v] = static_cast<typename E::scalar>(r1);
+                    const float x = op[u][c][v] * su;
+                    const _Float16 h = static_cast<_Float16>(x);
+                    xh[u][v] = h;
+                    xl[u][v] = static_cast<_Float16>(x - static_cast<float>(h));
                 }
-                else
+            }
+            else
+            {
+                // two values at a time: one packed round-to-nearest conversion per level, whose 32-bit word holds
+                // both bf16 parts, and their FP32 values read back from it (a bf16 is the top half of its float)
+#pragma unroll
+                for (int v = 0; v < 8; v += 2)
                 {
-                    const typename E::scalar mm = static_cast<typename E::scalar>(r1);
-                    xm[u][v] = mm;
-                    xl[u][v] = static_cast<typename E::scalar>(r1 - static_cast<float>(mm));
+                    const float x0 = op[u][c][v];
+                    const float x1 = op[u][c][v + 1];
+                    const bf16x2 h = __builtin_convertvector((float2v{x0, x1}), bf16x2);
+                    const uint32_t hb = __builtin_bit_cast(uint32_t, h);
+                    const float r0 = x0 - __builtin_bit_cast(float, hb << 16);
+                    const float r1 = x1 - __builtin_bit_cast(float, hb & 0xFFFF0000u);
+                    const bf16x2 m = __builtin_convertvector((float2v{r0, r1}), bf16x2);
+                    const uint32_t mb = __builtin_bit_cast(uint32_t, m);
+                    const bf16x2 lo = __builtin_convertvector(
+                        (float2v{r0 - __builtin_bit_cast(float, mb << 16), r1 - __builtin_bit_cast(float, mb & 0xFFFF0000u)}),
+                        bf16x2);
+                    xh[u][v] = h[0];
+                    xh[u][v + 1] = h[1];
+                    xm[u][v] = m[0];
+                    xm[u][v + 1] = m[1];
+                    xl[u][v] = lo[0];
+                    xl[u][v + 1] = lo[1];
                 }
             }
         }
