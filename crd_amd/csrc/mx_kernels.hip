@@ -368,9 +368,9 @@ __device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS]
                     const float r1 = x1 - __builtin_bit_cast(float, hb & 0xFFFF0000u);
                     const bf16x2 m = __builtin_convertvector((float2v{r0, r1}), bf16x2);
                     const uint32_t mb = __builtin_bit_cast(uint32_t, m);
-                    const bf16x2 lo = __builtin_convertvector(
-                        (float2v{r0 - __builtin_bit_cast(float, mb << 16), r1 - __builtin_bit_cast(float, mb & 0xFFFF0000u)}),
-                        bf16x2);
+                    const float l0 = r0 - __builtin_bit_cast(float, mb << 16);
+                    const float l1 = r1 - __builtin_bit_cast(float, mb & 0xFFFF0000u);
+                    const bf16x2 lo = __builtin_convertvector((float2v{l0, l1}), bf16x2);
                     xh[u][v] = h[0];
                     xh[u][v + 1] = h[1];
                     xm[u][v] = m[0];
