@@ -107,3 +107,22 @@ def test_split_precision_rejects_other_shapes(gpu):
         with pytest.raises(N_.CrdfftError) as e:
             gpu.FftPlan(n, 2, t, precision="split_fp16")
         assert e.value.status == N_.UNSUPPORTED
+
+
+@pytest.mark.parametrize("precision", ["split_bf16", "split_fp16"])
+def test_split_row_pass_walks_several_tiles(gpu, po, precision):
+    """A batch with more row-pass tiles (16 rows of one transform each) than the resident grid holds, so workgroups
+    walk several tiles and prefetch the next: every transform against FP64, out of place and in place."""
+    batch = 80  # 1280 row-pass tiles per chunk, more than the resident workgroups of one launch
+    x = wide_dynamic_range(batch, 91)
+    plan = gpu.FftPlan(N, batch, precision=precision)
+    src = gpu.DeviceBuffer.from_array(x)
+    dst = gpu.DeviceBuffer(x.nbytes)
+    plan.execute(src, dst, gpu.FORWARD)
+    plan.execute(src, None, gpu.FORWARD)
+    gpu.synchronize()
+    ref = po.fft64_of_f32(x)
+    for buf in (dst, src):
+        y = buf.download(x.shape, np.complex64)
+        per = [po.rel_l2(y[i], ref[i]) for i in range(batch)]
+        assert max(per) <= 1e-6, (precision, int(np.argmax(per)), max(per))
