@@ -1,6 +1,6 @@
 // Call sites written the way the reference's own tests write them (tests/numerics/hesap-fft/test_fft.cpp:83-84,
 // :138-139, :164-165, :235-237, :412-431, :523; bluestein / nd_fft / dct tests), over host memory, compiled against
-// include/crd_amd/fft.hpp with only the one-line namespace mapping a maintainer adds. The `crd`
+// include/crd_amd/fft.hpp with only the one-line namespace mapping a maintainer adds at the call site. The `crd`
 // namespace below stands in for the reference's own foundation types (allocator, spans, Complex, integer aliases):
 // just enough of their interface for these calls, no behaviour.
 //
