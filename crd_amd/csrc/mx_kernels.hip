@@ -526,9 +526,13 @@ __device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)
 
 // R = 16: the LDS tile allows four workgroups per CU (four waves per SIMD), so the registers are held to what four
 // waves per SIMD can have (128) — except in the row pass, whose prefetched next tile (32 more VGPRs) measured faster
-// at three waves per SIMD than spilled at four
+// at three waves per SIMD than spilled at four. R = 32: two workgroups per CU fit the LDS; the whole-transform pass
+// is held to two waves per SIMD (256 registers: BF16 then spills 8 and still measured 19 % faster than at one wave;
+// FP16 fits), the column and row passes reach two waves uncapped.
 template <int R, int PASS, int MODE, bool INV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 16 && PASS != 1 ? 4 : 1))) void k_mx(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 16 && PASS != 1 ? 4
+                                                                          : R == 32 && PASS == 2 ? 2
+                                                                                                 : 1))) void k_mx(
     SplitArgs a)
 {
     using M = Mx<R>;
@@ -655,9 +659,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
     else if constexpr (PASS == 1)
     {
         // rows row0 + q of the n1 x S matrix, read straight into the register layout (R consecutive points per
-        // lane group: 8R-byte runs). Tile = (transform, group of C rows); the launch either covers the tiles in one
-        // round (one per workgroup) or is a resident grid that walks them in steps of its size, each workgroup
-        // loading its next tile before it transforms the current one.
+        // lane group: 8R-byte runs). Tile = (transform, group of C rows). At R = 16 the launch is a resident grid
+        // that walks the tiles in steps of its size, each workgroup loading its next tile before it transforms the
+        // current one; at R = 32 it is one tile per workgroup with no prefetch (a second 1024-point tile in
+        // registers would halve the waves per SIMD).
+        constexpr bool walk = R == 16;
         constexpr int NQ = C / (kThreads / 64);
         const uint64_t tiles = a.count * groups;
         const uint64_t n1 = a.n_seq;
@@ -692,9 +698,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                     cur[u][r] = pre[u][r];
                 }
             }
-            if (tile + gridDim.x < tiles)
+            if constexpr (walk)
             {
-                load_tile(tile + gridDim.x, pre); // in flight while this tile is transformed and stored
+                if (tile + gridDim.x < tiles)
+                {
+                    load_tile(tile + gridDim.x, pre); // in flight while this tile is transformed and stored
+                }
             }
 #pragma unroll
             for (int u0 = 0; u0 < NQ; u0 += NS)
@@ -738,6 +747,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 const int k = k1 + R * k2;
                 const cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
                 dst[row0 + q + n1 * static_cast<uint64_t>(k)] = cf{v.x * a.scale, v.y * a.scale};
+            }
+            if constexpr (!walk)
+            {
+                break; // one tile per workgroup
             }
             __syncthreads(); // the tile's LDS reads are done before the next tile's writes
         }
