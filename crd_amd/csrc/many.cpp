@@ -505,7 +505,6 @@ crdfft_status crdfft_many_plan_create(crdfft_many_plan* plan, const uint64_t* le
     {
         return CRDFFT_NO_DEVICE;
     }
-    MANY_TRY(crdfft::many_prepare());
     auto* p = new (std::nothrow) crdfft_many_plan_s;
     if (p == nullptr)
     {

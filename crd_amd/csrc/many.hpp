@@ -70,9 +70,6 @@ struct ManyArgs
     const uint32_t* wg_begin; // workgroups + 1 entries
 };
 
-unsigned many_lds_bytes();
-// Raises the kernels' dynamic LDS limit (once per process, before any launch or capture).
-hipError_t many_prepare();
 hipError_t launch_many(const ManyArgs& a, unsigned workgroups, bool inverse, hipStream_t s);
 
 } // namespace crdfft

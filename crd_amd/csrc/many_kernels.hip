@@ -103,6 +103,18 @@ template <int R> __device__ __forceinline__ void twiddle_powers(cf (&w)[R])
     }
 }
 
+// The padded slot of point o (one pad slot per 16) as a byte offset, (o + o / 16) * 8, in unsigned arithmetic: a
+// shift and one add-and-shift per point.
+__device__ __forceinline__ uint32_t slot_bytes(uint32_t o) { return (o + (o >> 4)) << 3; }
+__device__ __forceinline__ cf* slot(cf* ld, uint32_t o)
+{
+    return reinterpret_cast<cf*>(reinterpret_cast<char*>(ld) + slot_bytes(o));
+}
+__device__ __forceinline__ const cf* slot(const cf* ld, uint32_t o)
+{
+    return reinterpret_cast<const cf*>(reinterpret_cast<const char*>(ld) + slot_bytes(o));
+}
+
 // v[m] = point (base + m * stride) of the tile for m < R. When the stride is a multiple of 16 the padded slots are
 // evenly spaced too (pad16(o + 16 s) = pad16(o) + 17 s): one address per butterfly and a scalar step.
 template <int R>
@@ -123,7 +135,7 @@ __device__ __forceinline__ void lds_gather(cf (&v)[R], const cf* ld, uint32_t ba
 #pragma unroll
         for (int m = 0; m < R; ++m)
         {
-            v[m] = ld[pad16(static_cast<int>(base + m * stride))];
+            v[m] = *slot(ld, base + m * stride);
         }
     }
 }
@@ -146,7 +158,7 @@ __device__ __forceinline__ void lds_scatter(const cf (&v)[R], cf* ld, uint32_t b
 #pragma unroll
         for (int m = 0; m < R; ++m)
         {
-            ld[pad16(static_cast<int>(base + m * stride))] = v[m];
+            *slot(ld, base + m * stride) = v[m];
         }
     }
 }
@@ -252,8 +264,10 @@ __device__ __forceinline__ void fetch_tile(const ManyArgs& a, uint32_t t, int ti
 
 template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs a)
 {
-    extern __shared__ float lds[];
-    cf* ld = reinterpret_cast<cf*>(lds);
+    // static LDS (the tile and the twiddle table, 141,440 bytes): a compile-time base, so the slot arithmetic needs no
+    // extra add for it
+    __shared__ cf lds_tile[kLdsPoints + kManyTwEntries];
+    cf* ld = lds_tile;
     cf* ltw = ld + kLdsPoints;
     const int tid = threadIdx.x;
     const uint32_t t0 = constant(a.wg_begin)[blockIdx.x];
@@ -321,23 +335,6 @@ template <bool INV> __global__ __launch_bounds__(kThreads) void k_many(ManyArgs 
 
 } // namespace
 
-unsigned many_lds_bytes() { return (kLdsPoints + kManyTwEntries) * 8u; }
-
-hipError_t many_prepare()
-{
-    for (auto kern : {k_many<false>, k_many<true>})
-    {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 static_cast<int>(many_lds_bytes()));
-        if (e != hipSuccess)
-        {
-            return e;
-        }
-    }
-    return hipSuccess;
-}
-
 hipError_t launch_many(const ManyArgs& a, unsigned workgroups, bool inverse, hipStream_t s)
 {
     if (workgroups == 0)
@@ -345,7 +342,7 @@ hipError_t launch_many(const ManyArgs& a, unsigned workgroups, bool inverse, hip
         return hipSuccess;
     }
     auto kern = inverse ? k_many<true> : k_many<false>;
-    hipLaunchKernelGGL(kern, dim3(workgroups), dim3(kThreads), many_lds_bytes(), s, a);
+    hipLaunchKernelGGL(kern, dim3(workgroups), dim3(kThreads), 0, s, a); // the tile is static LDS
     return hipGetLastError();
 }
 
