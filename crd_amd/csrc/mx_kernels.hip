@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "buffer.hpp"
 #include "dft.hpp"
 #include "mfma.hpp"
 
@@ -571,16 +572,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
             const int idx = tid_ / C + (kThreads / C) * m;
             return idx / R + R * (idx % R);
         };
+        // byte offsets within one transform (n * 8 bytes, below 4 GiB): a per-lane part and a uniform step per round
+        const uint32_t ncols32 = static_cast<uint32_t>(ncols);
         {
             // every load of the tile issued before any is waited for (S*C/kThreads per thread)
             constexpr int L = S * C / kThreads;
             const int q = tid % C;
             const int n0 = tid / C;
+            const __amdgpu_buffer_rsrc_t rin = make_rsrc(src);
+            const uint32_t lane = (static_cast<uint32_t>(n0) * ncols32 + static_cast<uint32_t>(col0) + q) * 8u;
+            const uint32_t step = (kThreads / C) * ncols32 * 8u;
             cf v[L];
 #pragma unroll
             for (int m = 0; m < L; ++m)
             {
-                v[m] = src[static_cast<uint64_t>(n0 + (kThreads / C) * m) * ncols + col0 + q];
+                v[m] = bld<0>(rin, lane, m * step);
             }
 #pragma unroll
             for (int m = 0; m < L; ++m)
@@ -620,25 +626,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
             }
         }
         __syncthreads();
-        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c
-        cf* dst = reinterpret_cast<cf*>(a.out) + t * a.odist;
+        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c; k = store_k(tid, m) = m + R (tid / C),
+        // so the byte offset is a per-lane part plus m * ncols * 8
+        static_assert(kThreads / C == R, "store round m of a thread is point m + R (tid / C)");
         const int q = tid % C;
         const uint64_t c = col0 + q;
+        const __amdgpu_buffer_rsrc_t rout = make_rsrc(reinterpret_cast<cf*>(a.out) + t * a.odist);
+        const uint32_t lane_k = (static_cast<uint32_t>(R * (tid / C)) * ncols32 + static_cast<uint32_t>(c)) * 8u;
+        const uint32_t kstep = ncols32 * 8u;
         if (a.tfull != nullptr)
         {
             // every twiddle load issued before the first is waited for
+            const __amdgpu_buffer_rsrc_t rtw = make_rsrc(a.tfull);
             cf twp[kStores];
 #pragma unroll
             for (int m = 0; m < kStores; ++m)
             {
-                twp[m] = reinterpret_cast<const cf*>(a.tfull)[static_cast<uint64_t>(store_k(tid, m)) * ncols + c];
+                twp[m] = bld<0>(rtw, lane_k, m * kstep);
             }
 #pragma unroll
             for (int m = 0; m < kStores; ++m)
             {
                 const int k = store_k(tid, m);
                 const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
-                dst[static_cast<uint64_t>(k) * ncols + c] = ctw<INV>(v, twp[m]);
+                bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, twp[m]));
             }
         }
         else
@@ -652,7 +663,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 const int k = store_k(tid, m);
                 const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
                 const uint64_t e = c * static_cast<uint64_t>(k);
-                dst[static_cast<uint64_t>(k) * ncols + c] = ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask]));
+                bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask])));
             }
         }
     }
