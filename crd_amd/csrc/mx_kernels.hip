@@ -66,6 +66,9 @@ template <> struct Mx<16>
     static constexpr int SP2 = S + S / 16;
     using acc_t = floatx4;
     __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+    // row(l, r) = row_lane(l) + row_reg(r)
+    __device__ static int row_lane(int l) { return 4 * (l >> 4); }
+    static constexpr int row_reg(int r) { return r; }
     // the 16-point blocks of every odd 64-point block swapped in pairs, so a wave's two 64-point halves (lane groups
     // 0 and 1 read points 64g + 16r + c) land on different banks without growing the tile
     __device__ static int pad(int o) { return o ^ (((o >> 6) & 1) << 4); }
@@ -84,6 +87,8 @@ template <> struct Mx<32>
     static constexpr int SP2 = SP;
     using acc_t = floatx16;
     __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+    __device__ static int row_lane(int l) { return 4 * (l >> 5); }
+    static constexpr int row_reg(int r) { return 8 * (r >> 2) + (r & 3); }
     __device__ static int pad(int o) { return o; }
     __device__ static int kidx(int l, int c, int v) { return 16 * c + 8 * (l >> 5) + v; }
 };
@@ -532,8 +537,8 @@ __device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)
 // FP16 fits), the column and row passes reach two waves uncapped.
 template <int R, int PASS, int MODE, bool INV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 16 && PASS != 1 ? 4
-                                                                          : R == 32 && PASS == 2 ? 2
-                                                                                                 : 1))) void k_mx(
+                                                                          : R == 32               ? 2
+                                                                                                  : 1))) void k_mx(
     SplitArgs a)
 {
     using M = Mx<R>;
@@ -678,18 +683,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
         constexpr int NQ = C / (kThreads / 64);
         const uint64_t tiles = a.count * groups;
         const uint64_t n1 = a.n_seq;
+        // byte offsets within one transform (n * 8 bytes, below 4 GiB): point R row(l, r) + row_l of row
+        // row0 + wave + u (kThreads / 64) splits into the lane's part and a uniform rest
+        const uint32_t lane_in = (static_cast<uint32_t>(wave) * S + R * M::row_lane(l) + row_l) * 8u;
         const auto load_tile = [&](uint64_t tile, cf (&dst)[NQ][P]) {
             const uint64_t tt = tile / groups;
-            const uint64_t row0 = (tile - tt * groups) * C;
-            const cf* src = reinterpret_cast<const cf*>(a.in) + tt * a.idist;
+            const uint32_t row0 = static_cast<uint32_t>(tile - tt * groups) * C;
+            const __amdgpu_buffer_rsrc_t rin = make_rsrc(reinterpret_cast<const cf*>(a.in) + tt * a.idist);
 #pragma unroll
             for (int u = 0; u < NQ; ++u)
             {
-                const cf* rp = src + (row0 + wave + u * (kThreads / 64)) * S;
 #pragma unroll
                 for (int r = 0; r < P; ++r)
                 {
-                    dst[u][r] = rp[R * M::row(l, r) + row_l];
+                    if constexpr (R == 16)
+                    {
+                        dst[u][r] = bld<0>(rin, lane_in, ((row0 + u * (kThreads / 64)) * S + R * M::row_reg(r)) * 8u);
+                    }
+                    else
+                    {
+                        // R = 32: the global form (the buffer form's schedule costs the BF16 pass 21 spilled VGPRs
+                        // at two waves per SIMD)
+                        dst[u][r] = (reinterpret_cast<const cf*>(a.in) + tt * a.idist)[
+                            (static_cast<uint64_t>(row0) + wave + u * (kThreads / 64)) * S + R * M::row(l, r) + row_l];
+                    }
                 }
             }
         };
@@ -743,21 +760,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 }
             }
             __syncthreads();
-            // point k of row row0 + q goes to out[row0 + q + n1 k]: C consecutive complex per k
+            // point k of row row0 + q goes to out[row0 + q + n1 k]: C consecutive complex per k; round m of a thread
+            // stores k = m + R (tid / C), at a per-lane byte offset plus (row0 + m n1) * 8
             const uint64_t tt = tile / groups;
-            const uint64_t row0 = (tile - tt * groups) * C;
-            cf* dst = reinterpret_cast<cf*>(a.out) + tt * a.odist;
+            const uint32_t row0 = static_cast<uint32_t>(tile - tt * groups) * C;
+            const __amdgpu_buffer_rsrc_t rout = make_rsrc(reinterpret_cast<cf*>(a.out) + tt * a.odist);
             const int q = tid % C;
             constexpr int per = kThreads / C;
+            static_assert(per == R, "store round m of a thread is point m + R (tid / C)");
+            const uint32_t n1_32 = static_cast<uint32_t>(n1);
+            const uint32_t lane_out = (static_cast<uint32_t>(q) + n1_32 * static_cast<uint32_t>(R * (tid / C))) * 8u;
 #pragma unroll 4
             for (int m = 0; m < S / per; ++m)
             {
                 const int idx = tid / C + per * m;
                 const int k1 = idx / R;
                 const int k2 = idx % R;
-                const int k = k1 + R * k2;
                 const cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
-                dst[row0 + q + n1 * static_cast<uint64_t>(k)] = cf{v.x * a.scale, v.y * a.scale};
+                bst<0>(rout, lane_out, (row0 + m * n1_32) * 8u, cf{v.x * a.scale, v.y * a.scale});
             }
             if constexpr (!walk)
             {
