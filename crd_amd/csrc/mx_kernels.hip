@@ -532,9 +532,9 @@ __device__ __forceinline__ void sequences(float (&re)[NS][Mx<R>::P], float (&im)
 
 // R = 16: the LDS tile allows four workgroups per CU (four waves per SIMD), so the registers are held to what four
 // waves per SIMD can have (128) — except in the row pass, whose prefetched next tile (32 more VGPRs) measured faster
-// at three waves per SIMD than spilled at four. R = 32: two workgroups per CU fit the LDS; the whole-transform pass
-// is held to two waves per SIMD (256 registers: BF16 then spills 8 and still measured 19 % faster than at one wave;
-// FP16 fits), the column and row passes reach two waves uncapped.
+// at three waves per SIMD than spilled at four. R = 32: two workgroups per CU fit the LDS, so every pass is held to
+// two waves per SIMD (256 registers, all VGPRs; the whole-transform BF16 pass then spills 8 and still measured 19 %
+// faster than at one wave, the others fit).
 template <int R, int PASS, int MODE, bool INV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 16 && PASS != 1 ? 4
                                                                           : R == 32               ? 2
