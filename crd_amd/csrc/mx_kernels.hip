@@ -569,14 +569,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
         const cf* src = reinterpret_cast<const cf*>(a.in) + t * a.idist;
         const uint64_t ncols = a.n_seq;
         const uint64_t col0 = g * C;
-        // store round m of thread tid writes point k = k1 + R k2 of column q = tid % C, where idx = tid / C + m *
-        // kThreads / C enumerates k2-major (k2 = idx % R, k1 = idx / R): the threads of a half wave take points k2
-        // apart by one, so their LDS reads (o = R k1 + k2) fall on distinct banks
+        // store round m of thread tid writes point k = k1 + R k2 of column q = tid % C with k1 = m, k2 = tid / C
+        // (kThreads / C = R): the threads of a half wave take points k2 apart by one, so their LDS reads
+        // (o = R k1 + k2) fall on distinct banks
         constexpr int kStores = S * C / kThreads;
-        const auto store_k = [](int tid_, int m) {
-            const int idx = tid_ / C + (kThreads / C) * m;
-            return idx / R + R * (idx % R);
-        };
         // byte offsets within one transform (n * 8 bytes, below 4 GiB): a per-lane part and a uniform step per round
         const uint32_t ncols32 = static_cast<uint32_t>(ncols);
         {
@@ -631,8 +627,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
             }
         }
         __syncthreads();
-        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c; k = store_k(tid, m) = m + R (tid / C),
-        // so the byte offset is a per-lane part plus m * ncols * 8
+        // X[k] of column c = col0 + q, times W_n^{c k}, stored at k * ncols + c; k = m + R (tid / C), so the byte
+        // offset is a per-lane part plus m * ncols * 8
         static_assert(kThreads / C == R, "store round m of a thread is point m + R (tid / C)");
         const int q = tid % C;
         const uint64_t c = col0 + q;
@@ -652,8 +648,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
 #pragma unroll
             for (int m = 0; m < kStores; ++m)
             {
-                const int k = store_k(tid, m);
-                const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
+                const cf v = lds[q * M::SP + M::pad(R * m + tid / C)]; // point k = m + R (tid / C)
                 bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, twp[m]));
             }
         }
@@ -665,8 +660,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
 #pragma unroll 4
             for (int m = 0; m < kStores; ++m)
             {
-                const int k = store_k(tid, m);
-                const cf v = lds[q * M::SP + M::pad(R * (k % R) + k / R)];
+                const int k = m + R * (tid / C);
+                const cf v = lds[q * M::SP + M::pad(R * m + tid / C)];
                 const uint64_t e = c * static_cast<uint64_t>(k);
                 bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, cmul(thi[e >> a.hshift], tlo[e & lmask])));
             }
@@ -773,10 +768,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
 #pragma unroll 4
             for (int m = 0; m < S / per; ++m)
             {
-                const int idx = tid / C + per * m;
-                const int k1 = idx / R;
-                const int k2 = idx % R;
-                const cf v = lds[q * M::SP + M::pad(R * k1 + k2)];
+                const cf v = lds[q * M::SP + M::pad(R * m + tid / C)]; // point k1 + R k2 with k1 = m, k2 = tid / C
                 bst<0>(rout, lane_out, (row0 + m * n1_32) * 8u, cf{v.x * a.scale, v.y * a.scale});
             }
             if constexpr (!walk)
