@@ -705,29 +705,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 }
             }
         };
-        cf pre[NQ][P];
-        uint64_t tile = blockIdx.x;
-        load_tile(tile, pre);
-#pragma unroll 1
-        for (; tile < tiles; tile += gridDim.x)
-        {
-            cf cur[NQ][P];
-#pragma unroll
-            for (int u = 0; u < NQ; ++u)
-            {
-#pragma unroll
-                for (int r = 0; r < P; ++r)
-                {
-                    cur[u][r] = pre[u][r];
-                }
-            }
-            if constexpr (walk)
-            {
-                if (tile + gridDim.x < tiles)
-                {
-                    load_tile(tile + gridDim.x, pre); // in flight while this tile is transformed and stored
-                }
-            }
+        // transform, stage through LDS and store one tile held in registers
+        const auto process = [&](uint64_t tile, const cf (&cur)[NQ][P]) {
 #pragma unroll
             for (int u0 = 0; u0 < NQ; u0 += NS)
             {
@@ -771,11 +750,46 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 const cf v = lds[q * M::SP + M::pad(R * m + tid / C)]; // point k1 + R k2 with k1 = m, k2 = tid / C
                 bst<0>(rout, lane_out, (row0 + m * n1_32) * 8u, cf{v.x * a.scale, v.y * a.scale});
             }
-            if constexpr (!walk)
+        };
+        uint64_t tile = blockIdx.x;
+        cf ta[NQ][P];
+        load_tile(tile, ta);
+        if constexpr (!walk)
+        {
+            process(tile, ta); // one tile per workgroup
+        }
+        else
+        {
+            // two register tiles in alternation (no copies between them): the next tile's loads are in flight
+            // while the current one is transformed and stored; the workgroup-uniform exits keep the barriers matched
+            const uint64_t step = gridDim.x;
+            cf tb2[NQ][P];
+#pragma unroll 1
+            for (;;)
             {
-                break; // one tile per workgroup
+                if (tile + step < tiles)
+                {
+                    load_tile(tile + step, tb2);
+                }
+                process(tile, ta);
+                tile += step;
+                if (tile >= tiles)
+                {
+                    break;
+                }
+                __syncthreads(); // the tile's LDS reads are done before the next tile's writes
+                if (tile + step < tiles)
+                {
+                    load_tile(tile + step, ta);
+                }
+                process(tile, tb2);
+                tile += step;
+                if (tile >= tiles)
+                {
+                    break;
+                }
+                __syncthreads();
             }
-            __syncthreads(); // the tile's LDS reads are done before the next tile's writes
         }
     }
     else
