@@ -1055,7 +1055,8 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
                 X(2048, 16, 4) X(4096, 16, 4)
 
 #define CRDFFT_ROWTR_LIST(X)                                                                                           \
-    X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(128, 8, 16) X(128, 16, 32) X(256, 16, 16)              \
+    X(16, 16, 16) X(32, 16, 16) X(64, 16, 16) X(128, 16, 16) X(128, 8, 16) X(128, 8, 32) X(128, 16, 32)               \
+        X(256, 16, 16)                                                                                                 \
         X(256, 16, 8) X(256, 8, 16) X(256, 16, 32) X(256, 8, 32) X(512, 16, 16) X(512, 16, 8) X(512, 8, 16)            \
             X(1024, 16, 16) X(1024, 16, 8) X(1024, 32, 16) X(2048, 16, 8) X(2048, 16, 4) X(4096, 16, 4)
 
@@ -1064,7 +1065,8 @@ __global__ void k_copy(const cf* in, cf* out, uint64_t n, uint64_t count, uint64
 
 // (N2, E, R) of the real-transform row pass (the four-step splits the planner gives half lengths 2^15 .. 2^24)
 #define CRDFFT_ROWTR_R2C_LIST(X)                                                                                       \
-    X(256, 16, 16) X(256, 16, 32) X(256, 8, 32) X(512, 8, 16) X(1024, 16, 8) X(2048, 16, 8) X(4096, 16, 4)
+    X(128, 8, 32) X(128, 16, 32) X(256, 16, 16) X(256, 16, 32) X(256, 8, 32) X(512, 8, 16) X(1024, 16, 8)             \
+        X(2048, 16, 8) X(4096, 16, 4)
 
 // (N1, E, C) of the persistent prefetching column pass
 #define CRDFFT_COL_PF_LIST(X)                                                                                          \

@@ -267,7 +267,10 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
         if (type == CRDFFT_R2C && cn == (uint64_t{1} << 15))
         {
             // the R2C row pass stores runs of tile2 / 2 bins: 32 rows per workgroup (runs of 16, 128 bytes) at 8
-            // values per thread measured 3.5 % faster than 16 rows for n = 2^16
+            // values per thread measured 3.5 % faster than 16 rows for n = 2^16; 256-point columns over 128-point
+            // rows (the C2C column pass of 2^16) 5 % faster than 128 x 256
+            s.n1 = 256;
+            s.n2 = 128;
             s.e2 = 8;
             s.tile2 = 32;
         }
@@ -382,8 +385,10 @@ crdfft_status choose_shape(uint64_t n, uint64_t batch, int32_t type, int32_t pre
         }
         // two pipelines keep two chunks in flight: each gets half the workspace budget. C2R, whose column pass reads
         // every bin twice (X[j] and X[h - j]), also runs half-size chunks: measured 3-11 % faster at n = 2^16..2^22;
-        // so does R2C from 1024-point columns (n = 2^20: 5 % faster; 2^22: equal)
-        const bool half = (s.hints & kHintTwoPipes) != 0 || type == CRDFFT_C2R || (type == CRDFFT_R2C && s.n1 >= 1024);
+        // so does R2C from 1024-point columns (n = 2^20: 5 % faster; 2^22: equal) and at n = 2^16 over its 256 x 128
+        // split (1.7 % faster)
+        const bool half = (s.hints & kHintTwoPipes) != 0 || type == CRDFFT_C2R ||
+                          (type == CRDFFT_R2C && (s.n1 >= 1024 || cn == (uint64_t{1} << 15)));
         const uint64_t budget = half ? kChunkBytes / 2 : kChunkBytes;
         s.chunk = std::max<uint64_t>(1, budget / per);
         if (tuning != nullptr && tuning->chunk != 0)

@@ -104,8 +104,8 @@ def test_planner_four_step_schedules():
     assert (d["n1"], d["n2"], d["e2"], d["chunk"], d["hints"]) == (256, 128, 8, 512, 3)
     d = crd_amd.describe(1 << 20, 64, "c2r")
     assert (d["chunk"], d["hints"]) == (32, 3)
-    d = crd_amd.describe(1 << 16, 4096, "r2c")
-    assert (d["n1"], d["n2"], d["e2"], d["tile2"], d["chunk"], d["hints"]) == (128, 256, 8, 32, 1024, 0)
+    d = crd_amd.describe(1 << 16, 4096, "r2c")  # 256 x 128 like C2R, 32 rows of 8 values, half-size chunks
+    assert (d["n1"], d["n2"], d["e2"], d["tile2"], d["chunk"], d["hints"]) == (256, 128, 8, 32, 512, 0)
     d = crd_amd.describe(1 << 20, 64, "r2c")  # 1024-point columns: half-size chunks
     assert (d["n1"], d["chunk"], d["hints"]) == (1024, 32, 0)
     # a tuned plan can switch the pipelines either way
