@@ -678,29 +678,39 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
         constexpr int NQ = C / (kThreads / 64);
         const uint64_t tiles = a.count * groups;
         const uint64_t n1 = a.n_seq;
-        // byte offsets within one transform (n * 8 bytes, below 4 GiB): point R row(l, r) + row_l of row
-        // row0 + wave + u (kThreads / 64) splits into the lane's part and a uniform rest
-        const uint32_t lane_in = (static_cast<uint32_t>(wave) * S + R * M::row_lane(l) + row_l) * 8u;
         const auto load_tile = [&](uint64_t tile, cf (&dst)[NQ][P]) {
             const uint64_t tt = tile / groups;
             const uint32_t row0 = static_cast<uint32_t>(tile - tt * groups) * C;
-            const __amdgpu_buffer_rsrc_t rin = make_rsrc(reinterpret_cast<const cf*>(a.in) + tt * a.idist);
-#pragma unroll
-            for (int u = 0; u < NQ; ++u)
+            if constexpr (R == 16)
             {
+                // byte offsets within one transform (n * 8 bytes, below 4 GiB): point R row(l, r) + row_l of row
+                // row0 + wave + u (kThreads / 64) splits into the lane's part and a uniform rest
+                const __amdgpu_buffer_rsrc_t rin = make_rsrc(reinterpret_cast<const cf*>(a.in) + tt * a.idist);
+                const uint32_t lane_in = (static_cast<uint32_t>(wave) * S + R * M::row_lane(l) + row_l) * 8u;
 #pragma unroll
-                for (int r = 0; r < P; ++r)
+                for (int u = 0; u < NQ; ++u)
                 {
-                    if constexpr (R == 16)
+#pragma unroll
+                    for (int r = 0; r < P; ++r)
                     {
-                        dst[u][r] = bld<0>(rin, lane_in, ((row0 + u * (kThreads / 64)) * S + R * M::row_reg(r)) * 8u);
+                        dst[u][r] =
+                            bld<0>(rin, lane_in, ((row0 + u * (kThreads / 64)) * S + R * M::row_reg(r)) * 8u);
                     }
-                    else
+                }
+            }
+            else
+            {
+                // R = 32: global loads (the buffer form's schedule costs the BF16 pass 21 spilled VGPRs at two waves
+                // per SIMD)
+                const cf* src = reinterpret_cast<const cf*>(a.in) + tt * a.idist;
+#pragma unroll
+                for (int u = 0; u < NQ; ++u)
+                {
+#pragma unroll
+                    for (int r = 0; r < P; ++r)
                     {
-                        // R = 32: the global form (the buffer form's schedule costs the BF16 pass 21 spilled VGPRs
-                        // at two waves per SIMD)
-                        dst[u][r] = (reinterpret_cast<const cf*>(a.in) + tt * a.idist)[
-                            (static_cast<uint64_t>(row0) + wave + u * (kThreads / 64)) * S + R * M::row(l, r) + row_l];
+                        dst[u][r] = src[(static_cast<uint64_t>(row0) + wave + u * (kThreads / 64)) * S +
+                                        R * M::row(l, r) + row_l];
                     }
                 }
             }
