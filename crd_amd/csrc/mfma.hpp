@@ -22,13 +22,14 @@ struct SplitArgs
     const cfloat* tws;   // W_S^e, e < S: the stage twiddles of the sequence length S
     const cfloat* thi;   // pass 0: four-step twiddle W_n^{j << hshift}
     const cfloat* tlo;   // pass 0: W_n^{j}, j < 2^hshift
-    const cfloat* tfull; // pass 0, when set: W_n^{c k} at [k * n2 + c] (one coalesced load per value, n <= 2^18)
+    const cfloat* tfull; // pass 0, when set: W_n^{c k} at [k * n2 + c] (one coalesced load per value)
     uint32_t hshift;
     uint64_t count;      // transforms (pass 2) or four-step matrices (passes 0, 1)
     uint64_t idist;
     uint64_t odist;
     uint64_t n_seq;      // pass 0: columns (n2); pass 1: rows (n1); pass 2: 1
     float scale;         // pass 1 and 2: applied to every output
+    const cfloat* tcm;   // pass 0, when set and tfull is not: W_n^{c m} at [m * n2 + c] for m < R
 };
 
 // The radix of a sequence length (256 -> 16, 1024 -> 32), false for other lengths.

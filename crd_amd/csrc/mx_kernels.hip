@@ -652,6 +652,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R == 1
                 bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, twp[m]));
             }
         }
+        else if (a.tcm != nullptr)
+        {
+            // W_n^{c k} for k = m + R (tid / C) as W_n^{c R (tid / C)} (two table gathers, once per thread) times
+            // W_n^{c m} from the R-row table (one coalesced load per value from a table the L2 holds)
+            const cf* thi = reinterpret_cast<const cf*>(a.thi);
+            const cf* tlo = reinterpret_cast<const cf*>(a.tlo);
+            const uint64_t lmask = (uint64_t{1} << a.hshift) - 1;
+            const uint64_t e0 = c * static_cast<uint64_t>(R * (tid / C));
+            const cf w0 = cmul(thi[e0 >> a.hshift], tlo[e0 & lmask]);
+            const __amdgpu_buffer_rsrc_t rcm = make_rsrc(a.tcm);
+            const uint32_t lane_c = static_cast<uint32_t>(c) * 8u;
+            cf twp[kStores];
+#pragma unroll
+            for (int m = 0; m < kStores; ++m)
+            {
+                twp[m] = bld<0>(rcm, lane_c, m * kstep);
+            }
+#pragma unroll
+            for (int m = 0; m < kStores; ++m)
+            {
+                const cf v = lds[q * M::SP + M::pad(R * m + tid / C)];
+                bst<0>(rout, lane_k, m * kstep, ctw<INV>(v, cmul(w0, twp[m])));
+            }
+        }
         else
         {
             const cf* thi = reinterpret_cast<const cf*>(a.thi);

@@ -151,6 +151,7 @@ struct crdfft_plan_s
     // split-precision MFMA passes: operand splits of each pass's radix per direction ([0] column or single pass,
     // [1] row pass) and the stage twiddles W_S of each pass's sequence length
     DeviceBuffer split_fwd[2], split_inv[2], split_tw[2];
+    DeviceBuffer split_tcm; // radix-32 column pass: W_n^{c m} at [m * n2 + c], m < 32
     // Bluestein: the power-of-two plan of m, chirp c[k], FFT_m(b), and the chunk workspace
     crdfft_plan inner = nullptr;
     crdfft_many_plan many = nullptr; // Bluestein plans of a 2/3/5/7-smooth n: the mixed-radix launch instead
@@ -285,10 +286,19 @@ crdfft_status build_tables(crdfft_plan_s* p, hipStream_t s)
             CRDFFT_TRY(p->tlo.alloc(lo * sizeof(cfloat)));
             CRDFFT_TRY(crdfft::build_roots(p->thi.as<cfloat>(), hi, lo, cn, s));
             CRDFFT_TRY(crdfft::build_roots(p->tlo.as<cfloat>(), lo, 1, cn, s));
-            // the full table W_n^{c k} at [k * n2 + c] for every split length (n <= 2^20, 8 MiB): one coalesced
-            // load per stored value instead of two table gathers (measured 15-18 % faster at 2^20)
-            CRDFFT_TRY(p->tfull.alloc(cn * sizeof(cfloat)));
-            CRDFFT_TRY(crdfft::build_full_twiddle(p->tfull.as<cfloat>(), sh.n1, sh.n2, cn, s));
+            if (sh.n1 == 1024)
+            {
+                // radix-32 column pass (n = 2^20): W_n^{c k} = W_n^{c 32 (tid / 8)} W_n^{c m}, the second factor
+                // from a 32-row table (256 KiB) that stays in the L2
+                CRDFFT_TRY(p->split_tcm.alloc(32 * sh.n2 * sizeof(cfloat)));
+                CRDFFT_TRY(crdfft::build_full_twiddle(p->split_tcm.as<cfloat>(), 32, sh.n2, cn, s));
+            }
+            else
+            {
+                // the full table W_n^{c k} at [k * n2 + c]: one coalesced load per stored value
+                CRDFFT_TRY(p->tfull.alloc(cn * sizeof(cfloat)));
+                CRDFFT_TRY(crdfft::build_full_twiddle(p->tfull.as<cfloat>(), sh.n1, sh.n2, cn, s));
+            }
             CRDFFT_TRY(p->workspace.alloc(sh.workspace_bytes));
         }
         return CRDFFT_OK;
@@ -642,9 +652,9 @@ crdfft_status execute_impl(crdfft_plan p, const void* in, void* out, int32_t dir
         for (uint64_t c0 = 0; c0 < sh.batch; c0 += sh.chunk)
         {
             const uint64_t cnt = std::min(sh.chunk, sh.batch - c0);
-            const crdfft::SplitArgs a{x + c0 * p->idist, ws, tab0, p->split_tw[0].as<cfloat>(), p->thi.as<cfloat>(),
-                                      p->tlo.as<cfloat>(), p->tfull.as<cfloat>(), p->hshift, cnt, p->idist,
-                                      sh.complex_n, sh.n2, 1.0f};
+            const crdfft::SplitArgs a{x + c0 * p->idist, ws,        tab0, p->split_tw[0].as<cfloat>(), p->thi.as<cfloat>(),
+                                      p->tlo.as<cfloat>(), p->tfull.as<cfloat>(), p->hshift, cnt,  p->idist,
+                                      sh.complex_n,        sh.n2,     1.0f, p->split_tcm.as<cfloat>()};
             CRDFFT_TRY(prof.before());
             CRDFFT_TRY(crdfft::launch_split_pass(0, sh.tile1, sh.precision, inverse, a, s));
             CRDFFT_TRY(prof.after(0, sh.complex_n * cnt));
@@ -1308,7 +1318,7 @@ crdfft_status crdfft_plan_get_info(crdfft_plan p, crdfft_plan_info* info)
     info->check = p->check;
     info->table_bytes = p->tw_single.bytes + p->tw1.bytes + p->tw2.bytes + p->thi.bytes + p->tlo.bytes +
                         p->tstep.bytes + p->tfull.bytes + p->wreal.bytes + p->split_fwd[0].bytes + p->split_inv[0].bytes + p->split_tw[0].bytes +
-                        p->split_fwd[1].bytes + p->split_inv[1].bytes + p->split_tw[1].bytes;
+                        p->split_fwd[1].bytes + p->split_inv[1].bytes + p->split_tw[1].bytes + p->split_tcm.bytes;
     return CRDFFT_OK;
 }
 
