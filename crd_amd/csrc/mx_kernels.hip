@@ -46,6 +46,7 @@ namespace
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -352,12 +353,18 @@ __device__ __forceinline__ void stage(float (&re)[NS][Mx<R>::P], float (&im)[NS]
             {
                 const float su = ldexpf(1.0f, sx[u]);
 #pragma unroll
-                for (int v = 0; v < 8; ++v)
+                for (int v = 0; v < 8; v += 2)
                 {
-                    const float x = op[u][c][v] * su;
-                    const _Float16 h = static_cast<_Float16>(x);
-                    xh[u][v] = h;
-                    xl[u][v] = static_cast<_Float16>(x - static_cast<float>(h));
+                    // two values at a time: one packed conversion per part (su is a power of two: x is exact)
+                    const float x0 = op[u][c][v] * su;
+                    const float x1 = op[u][c][v + 1] * su;
+                    const half2v h = __builtin_convertvector((float2v{x0, x1}), half2v);
+                    const half2v lo = __builtin_convertvector(
+                        (float2v{x0 - static_cast<float>(h[0]), x1 - static_cast<float>(h[1])}), half2v);
+                    xh[u][v] = h[0];
+                    xh[u][v + 1] = h[1];
+                    xl[u][v] = lo[0];
+                    xl[u][v + 1] = lo[1];
                 }
             }
             else
