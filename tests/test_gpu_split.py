@@ -126,3 +126,23 @@ def test_split_row_pass_walks_several_tiles(gpu, po, precision):
         y = buf.download(x.shape, np.complex64)
         per = [po.rel_l2(y[i], ref[i]) for i in range(batch)]
         assert max(per) <= 1e-6, (precision, int(np.argmax(per)), max(per))
+
+
+@pytest.mark.parametrize("precision", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("n,batch,chunk", [(1 << 16, 7, 3), (1 << 18, 5, 2), (1 << 20, 3, 2)])
+def test_split_chunks(gpu, po, precision, n, batch, chunk):
+    """Several chunks through one workspace, the last one partial (the default chunks of 2^16 and 2^18 hold 256 and
+    64 transforms, more than the other tests' batches): every transform against FP64, both directions."""
+    x = wide_dynamic_range(batch, 7 * n + batch, n)
+    plan = gpu.FftPlan(n, batch, precision=precision, tuning={"chunk": chunk})
+    assert plan.info()["chunk"] == chunk
+    src = gpu.DeviceBuffer.from_array(x)
+    dst = gpu.DeviceBuffer(x.nbytes)
+    for inverse in (False, True):
+        plan.execute(src, dst, gpu.INVERSE if inverse else gpu.FORWARD)
+        gpu.synchronize()
+        y = dst.download(x.shape, np.complex64)
+        ref = po.fft64_of_f32(x, inverse=inverse)
+        per = [po.rel_l2(y[i], ref[i]) for i in range(batch)]
+        print(f"{precision} n={n} chunk={chunk} inverse={inverse} max rel_l2={max(per):.3e}")
+        assert max(per) <= 1e-6, (precision, n, inverse, int(np.argmax(per)), max(per))
