@@ -235,8 +235,9 @@ def test_r2c_c2r_padded_rows(gpu, po, n, batch, pad_in, pad_out):
 
 
 def test_real_four_step_over_several_chunks(gpu, po):
-    """n = 2^16 real (half length 32768 on the four-step, 1024 transforms per 256 MiB chunk) over 1030 rows: the
-    fused row pass (R2C) and fused column pass (C2R) across a chunk boundary, checked on rows either side of it."""
+    """n = 2^16 real (half length 32768 on the four-step, 512 transforms per 128 MiB chunk in both directions) over
+    1030 rows: the fused row pass (R2C) and fused column pass (C2R) across two chunk boundaries and a partial last
+    chunk, checked on rows either side of each boundary."""
     n, batch = 1 << 16, 1030
     h = n // 2
     x = po.normal_real(n * batch, 77).reshape(batch, n).astype(np.float32)
@@ -249,7 +250,7 @@ def test_real_four_step_over_several_chunks(gpu, po):
     r2c.execute(xin, spec, gpu.FORWARD)
     c2r.execute(spec, back, gpu.INVERSE)
     gpu.synchronize()
-    rows = [0, 1, 1022, 1023, 1024, 1025, batch - 1]
+    rows = [0, 1, 511, 512, 1022, 1023, 1024, 1025, batch - 1]
     X = spec.download((batch, h + 1), np.complex64)[rows]
     assert po.rel_l2(X, po.rfft64(x[rows].astype(np.float64))) <= bound(n)
     xb = back.download((batch, n), np.float32)[rows]
